@@ -1,0 +1,172 @@
+"""Benchmark of the HIP rendering core on the bvh_stress workload.
+
+Workload (one "step" = one full frame): the seeded bvh_stress scene -- a
+displaced geodesic icosphere of 1,048,820 triangles plus 4,096 random spheres
+(1,052,916 primitives, reference-topology SAH BVH) -- at 1920x1080, 1 spp,
+Whitted depth 1 with one hard point light: one primary (closest-hit) ray and
+one shadow (any-hit) ray per pixel that hits.  Scene, BVH and output buffer
+are resident in HBM before the timed region.
+
+value = algorithmic Mrays/s (closest-hit + any-hit queries per second, whole
+frame, host wall clock around K synchronous frames).  The reference's own
+convention (trace() calls only, no shadow rays) is reported beside it.
+
+roofline: the dominant kernel is the render megakernel.  Its algorithmic bytes
+per launch are sum over queries of (32 B ray + 8 B hit) + 64 B per internal
+node entered + the primitive's bytes per test (sphere 16, triangle 48, cube 16,
+plane 32, cylinder 48, cone 48), counted by an instrumented launch of the
+same frame; divided by the kernel's mean HIP-event duration.
+
+cpu_baseline: the reference renderer (oracle/_ref/pencil, built from the
+reference sources) on the same scene written as .pecl + .obj, all host cores
+used (capped at 16), render() time as the reference prints it; falls back to
+the oracle port if the reference binary was not built.
+"""
+import argparse
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PRIM_BYTES = [16, 48, 16, 32, 48, 48]   # sphere, triangle, cube, plane, cylinder, cone
+HBM_PEAK_GBS = 8000.0
+STRESS = dict(freq=229, n_spheres=4096, seed=1234, width=1920, height=1080)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(queries, threads):
+    """Reference renderer on the same frame; returns dict or None."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "pencil")
+    import pencil_amd as pa
+    tmp = tempfile.mkdtemp(prefix="pcl_bench_")
+    try:
+        if os.path.exists(ref):
+            pa.HostScene.write_stress(tmp, "stress", **STRESS)
+            out = os.path.join(tmp, "out.ppm")
+            t0 = time.time()
+            r = subprocess.run([ref, os.path.join(tmp, "stress.pecl"), out, "--threads", str(threads)],
+                               capture_output=True, text=True, timeout=600)
+            wall = time.time() - t0
+            m = re.search(r"rendered in ([0-9.]+) s \(([0-9,]+) rays", r.stdout)
+            if r.returncode != 0 or not m:
+                log("reference run failed:", r.returncode, r.stderr[-500:])
+                return None
+            sec = float(m.group(1))
+            return {"value": round(queries / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                    "kind": "reference",
+                    "sample": f"full 1920x1080 frame, reference binary built from the reference sources, "
+                              f"render() {sec:.3f} s as it prints it ({m.group(2)} reference-convention rays; "
+                              f"{wall:.1f} s wall incl. .pecl/.obj parse and BVH build)"}
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        hs = pa.HostScene.stress(**STRESS)
+        osc = ol.OracleScene(hs.desc())
+        s = hs.settings()
+        rows = (0, 270)   # a quarter of the frame
+        t0 = time.time()
+        _, c = osc.render(hs.camera(), s, threads=threads, rows=rows)
+        sec = time.time() - t0
+        q = c.closest_queries + c.anyhit_queries
+        return {"value": round(q / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                "sample": f"rows 0-269 of the 1920x1080 frame through the oracle ({q} queries, {sec:.2f} s)"}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the path would shard pixels or scenes across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import pencil_amd as pa
+    t0 = time.time()
+    hs = pa.HostScene.stress(**STRESS)
+    desc, cam, st = hs.desc(), hs.camera(), hs.settings()
+    log(f"scene: {desc.n_prims} primitives, generated in {time.time() - t0:.2f} s")
+    core = pa.Core(0)
+    sid = core.upload(desc)
+    info = core.scene_info(sid)
+    log(f"bvh: {info.n_nodes} nodes, depth {info.depth}, {info.device_bytes / 1e6:.1f} MB on device, "
+        f"build {info.build_ms:.0f} ms, upload {info.upload_ms:.0f} ms")
+    W, H = st.width, st.height
+    d_out = core.alloc(W * H * 3 * 4)
+
+    # instrumented frame: work counts for the bytes model (not timed)
+    cst, wc = core.render_counts(sid, cam, st, d_out)
+    queries = cst.closest_queries + cst.anyhit_queries
+    tests = [wc.prim_tests[k] for k in range(6)]
+    bytes_per_launch = queries * (32 + 8) + 64 * wc.node_visits + sum(b * t for b, t in zip(PRIM_BYTES, tests))
+    log(f"counts: {cst.closest_queries} closest + {cst.anyhit_queries} any-hit queries, "
+        f"{wc.node_visits} node visits, tests {tests}, {bytes_per_launch / 1e9:.2f} GB algorithmic per frame")
+
+    for _ in range(args.warmup):
+        core.render_device(sid, cam, st, d_out)
+    kms = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s = core.render_device(sid, cam, st, d_out)
+        kms.append(s.kernel_ms)
+    t_end = time.perf_counter()
+    ms_per_step = (t_end - t_start) * 1000.0 / args.steps
+    kernel_ms = sum(kms) / len(kms)
+    value = queries / (ms_per_step / 1000.0) / 1e6
+    achieved = bytes_per_launch / (kernel_ms / 1000.0) / 1e9
+    log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame wall, {kernel_ms:.3f} ms kernel (HIP events)")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(queries, threads)
+        except Exception as e:   # the baseline is reported, never required
+            log("cpu baseline failed:", e)
+    core.free(d_out)
+    core.free_scene(sid)
+    core.close()
+
+    line = {
+        "metric": "Mrays/s at 1080p 1M-tri (plain vs batched); adjoint grad-steps/s at FD tol",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded bvh_stress scene)",
+        "config": {"workload": "bvh_stress: 1,048,820-tri displaced icosphere + 4,096 spheres, 1920x1080, "
+                               "1 spp, depth 1, primary + 1 shadow ray (plain per-pixel megakernel)",
+                   "primitives": desc.n_prims, "width": W, "height": H, "spp": 1,
+                   "queries_per_frame": queries, "ref_convention_rays_per_frame": cst.rays_traced,
+                   "ref_convention_mrays_s": round(cst.rays_traced / (ms_per_step / 1000.0) / 1e6, 3),
+                   "kernel_ms": round(kernel_ms, 4), "bvh_build_ms": round(info.build_ms, 1),
+                   "path": "plain"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,230 @@
+/*
+ * pencil_hip.h - C ABI of the MI355X (gfx950) rendering core for Pencil.
+ *
+ * The core replaces the data-parallel hot path behind the reference's
+ * Renderer::render: SAH-BVH traversal, ray-primitive intersection (sphere,
+ * triangle/quad, cube, cylinder, cone, plane), Whitted shading and path tracing
+ * with next-event estimation.  Everything above this header (the .pecl parser,
+ * Scene/Camera/Material, the CLI) stays on the host side of the caller.
+ *
+ * Conventions
+ *   - Plain C, fixed-size fields, no C++ or torch types.
+ *   - Every function returns PCL_OK (0) or a negative PCL_ERR_* code; the message
+ *     is available from pcl_last_error(ctx).  Nothing throws across the ABI.
+ *   - Calls are synchronous: work is enqueued on the context's HIP stream and the
+ *     call returns when the stream is idle.  One host thread per context.
+ *   - The caller owns every host buffer it passes; the library owns device
+ *     buffers it allocates until pcl_scene_free / pcl_destroy.  No pointer is
+ *     kept past a call.
+ *   - Colors are linear HDR float RGB, row-major, row 0 = top of the image, the
+ *     layout of the reference's std::vector<Color> (src/renderer.hpp:198-201).
+ */
+#ifndef PENCIL_HIP_H
+#define PENCIL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCL_ABI_VERSION 1
+
+enum {
+    PCL_OK              = 0,
+    PCL_ERR_INVALID     = -1,  /* bad argument / handle */
+    PCL_ERR_HIP         = -2,  /* a HIP runtime call failed */
+    PCL_ERR_OOM         = -3,  /* device allocation failed */
+    PCL_ERR_UNSUPPORTED = -4,  /* scene uses a feature outside this core */
+    PCL_ERR_LIMIT       = -5   /* a size limit of this core was exceeded */
+};
+
+/* ---- scene description (host side, insertion order) -------------------- */
+
+/* Primitive kinds: the six in-scope Primitive subclasses of
+ * src/primitives.hpp (quad = two triangles, src/primitives.hpp:390-396). */
+enum {
+    PCL_PRIM_SPHERE   = 0,  /* p[0..2] center, p[3] radius            (primitives.hpp:92-134)  */
+    PCL_PRIM_TRIANGLE = 1,  /* p[0..2] v0, p[3..5] v1, p[6..8] v2     (primitives.hpp:137-192) */
+    PCL_PRIM_CUBE     = 2,  /* p[0..2] center, p[3] half extent       (primitives.hpp:195-252) */
+    PCL_PRIM_PLANE    = 3,  /* p[0..2] point, p[3..5] normal (raw)    (primitives.hpp:255-284) */
+    PCL_PRIM_CYLINDER = 4,  /* p[0..2] center, p[3..5] axis (raw), p[6] radius, p[7] height (primitives.hpp:399-505) */
+    PCL_PRIM_CONE     = 5   /* p[0..2] apex, p[3..5] axis (raw), p[6] radius, p[7] height   (primitives.hpp:508-606) */
+};
+
+/* Constructor arguments of one primitive, exactly as the reference's
+ * constructor receives them (derived values such as normalized axes are
+ * computed by the library the way the constructors do). */
+typedef struct pcl_prim {
+    int32_t kind;
+    int32_t material_id;
+    float   p[12];
+} pcl_prim;
+
+enum { PCL_MAT_DIFFUSE = 0, PCL_MAT_METAL = 1, PCL_MAT_DIELECTRIC = 2, PCL_MAT_PRINCIPLED = 3 };
+
+/* The shading fields of Material (src/material.hpp:29-80) the core uses.
+ * Texture, normal map, displacement, SSS, dispersion and principled fields
+ * are outside the core; a scene that needs them is rejected. */
+typedef struct pcl_material {
+    int32_t kind;
+    float   albedo[3];
+    float   reflectivity;
+    float   ior;
+    float   specular;
+    float   shininess;
+    float   specular_color[3];
+    float   emission[3];
+} pcl_material;
+
+enum { PCL_LIGHT_POINT = 0, PCL_LIGHT_DIRECTIONAL = 1 };
+
+/* Light (src/light.hpp:20-26). */
+typedef struct pcl_light {
+    int32_t kind;
+    float   pos_or_dir[3];
+    float   intensity[3];
+    float   radius;
+    int32_t is_emissive;
+} pcl_light;
+
+typedef struct pcl_scene_desc {
+    const pcl_prim*     prims;      /* insertion order (Scene::add_primitive) */
+    int32_t             n_prims;
+    const pcl_material* materials;
+    int32_t             n_materials;
+    const pcl_light*    lights;
+    int32_t             n_lights;
+    float               background[3];
+    float               ambient[3];
+    int32_t             use_bvh;    /* Scene::use_bvh (src/scene.hpp:53) */
+} pcl_scene_desc;
+
+/* Camera constructor arguments (src/camera.hpp:29-36, camera.cpp:48-68). */
+typedef struct pcl_camera {
+    float   eye[3];
+    float   look[3];
+    float   up[3];
+    float   vfov_deg;
+    float   aspect;
+    float   aperture;
+    float   focus_dist;
+    int32_t bokeh_blades;
+} pcl_camera;
+
+/* The RenderSettings fields (src/renderer.hpp:34-83) on the core's path. */
+typedef struct pcl_settings {
+    int32_t width;
+    int32_t height;
+    int32_t samples;            /* n: n*n stratified jittered samples per pixel */
+    int32_t max_depth;          /* Whitted recursion depth */
+    float   shadow_bias;        /* default 1e-3 */
+    int32_t soft_shadow_samples;/* default 4 */
+    int32_t pathtrace;          /* 0 = Whitted trace(), 1 = trace_path() */
+    int32_t pathtrace_bounces;  /* default 4 */
+} pcl_settings;
+
+typedef struct pcl_stats {
+    uint64_t rays_traced;       /* reference convention: trace()/trace_path() calls */
+    uint64_t closest_queries;   /* closest-hit BVH queries cast */
+    uint64_t anyhit_queries;    /* any-hit (shadow) BVH queries cast */
+    double   kernel_ms;         /* HIP-event time of the launches on the context stream */
+    double   total_ms;          /* host wall time of the call */
+} pcl_stats;
+
+/* Ray batches for the traversal entry point. */
+typedef struct pcl_ray {
+    float origin[3];
+    float dir[3];
+    float tmin;
+    float tmax;
+} pcl_ray;
+
+typedef struct pcl_hit {
+    float   t;          /* hit distance, or -1 on a miss */
+    int32_t prim_id;    /* insertion index of the primitive that Scene::hit
+                           returns (any-hit: 1 occluded / -1 clear) */
+} pcl_hit;
+
+typedef struct pcl_scene_info {
+    int32_t n_prims;
+    int32_t n_nodes;        /* internal BVH nodes (64 B each) */
+    int32_t depth;          /* longest root-to-leaf path, in edges */
+    int32_t root;           /* >=0 internal node, <0 leaf ~slot, INT32_MIN empty */
+    uint64_t device_bytes;  /* nodes + primitives + materials + lights */
+    double   build_ms;      /* host BVH build time */
+    double   upload_ms;     /* host-to-device upload time */
+} pcl_scene_info;
+
+typedef struct pcl_ctx pcl_ctx;
+
+/* Context: one HIP device, one stream.  Replaces nothing in the reference
+ * (which has no device); owned by the caller of Renderer::render. */
+int         pcl_init(int device, pcl_ctx** out);
+void        pcl_destroy(pcl_ctx* ctx);
+const char* pcl_last_error(const pcl_ctx* ctx);
+int         pcl_abi_version(void);
+
+/* Scene upload: builds the SAH BVH with the reference's topology
+ * (BVH::build, src/bvh.hpp:54-197) on the host and uploads the flattened
+ * 64-byte nodes, primitives in BVH leaf order, materials and lights.
+ * Replaces ParsedScene::finalize -> Scene::build_bvh (src/scene.hpp:48-51)
+ * for the device.  *scene_out receives a handle >= 0. */
+int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_out);
+int pcl_scene_free(pcl_ctx* ctx, int32_t scene);
+int pcl_scene_info_get(pcl_ctx* ctx, int32_t scene, pcl_scene_info* info);
+/* BVH storage order: out[slot] = insertion index of the primitive in leaf
+ * `slot` (the order of BVH::storage_ after build, src/bvh.hpp:54-58). */
+int pcl_scene_leaf_order(pcl_ctx* ctx, int32_t scene, int32_t* out, int32_t n);
+/* The host half of pcl_scene_upload alone, needing no device: validates the
+ * description, builds the BVH and reports its shape (info) and leaf order
+ * (leaf_order[slot] = insertion id; n >= n_prims).  Lets a caller check the
+ * topology against its own BVH::build without a GPU. */
+int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t* leaf_order, int32_t n);
+
+/* Ray-batch traversal: Scene::hit (closest, any_hit = 0) or Scene::occluded
+ * (any_hit = 1), src/scene.hpp:55-81.  Host buffers. */
+int pcl_trace_rays(pcl_ctx* ctx, int32_t scene, const pcl_ray* rays, int32_t n,
+                   int32_t any_hit, pcl_hit* hits, pcl_stats* stats);
+/* Same, with rays and hits already in device memory. */
+int pcl_trace_rays_device(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, int32_t n,
+                          int32_t any_hit, pcl_hit* d_hits, pcl_stats* stats);
+
+/* Work counters of an instrumented launch: internal BVH nodes entered and
+ * primitive tests per PCL_PRIM_* kind, summed over the launch.  They size
+ * the algorithmic bytes of the work (64 B per node, the primitive's bytes
+ * per test).  No reference counterpart: the reference counts neither. */
+typedef struct pcl_work_counts {
+    uint64_t node_visits;
+    uint64_t prim_tests[6];
+} pcl_work_counts;
+
+/* pcl_trace_rays_device with the work counters (a separate, instrumented
+ * kernel variant; the plain entry points run uninstrumented). */
+int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
+                          pcl_hit* d_hits, pcl_work_counts* counts);
+
+/* Renderer::render (src/renderer.hpp:198-295) for the in-scope settings:
+ * per-pixel xorshift stream, n*n jittered samples, trace() (Whitted,
+ * :403-822) or trace_path() (NEE, :829-972).  out_rgb: W*H*3 floats (host). */
+int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam,
+               const pcl_settings* settings, float* out_rgb, pcl_stats* stats);
+/* Same, writing into a device buffer of W*H*3 floats. */
+int pcl_render_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam,
+                      const pcl_settings* settings, float* d_out_rgb, pcl_stats* stats);
+/* pcl_render_device through the instrumented kernel variant, with counts. */
+int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                      float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts);
+
+/* Device-memory helpers for callers without their own allocator. */
+int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr);
+int pcl_device_free(pcl_ctx* ctx, void* d_ptr);
+int pcl_memcpy_h2d(pcl_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int pcl_memcpy_d2h(pcl_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PENCIL_HIP_H */

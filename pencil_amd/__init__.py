@@ -1,0 +1,343 @@
+"""pencil_amd - Python bindings of the MI355X (gfx950) rendering core for Pencil.
+
+The product is the C ABI in ``include/pencil_hip.h`` (``libpencil_hip.so``:
+hand-written HIP kernels plus the host-side BVH build).  This module binds it
+with ctypes, together with ``libpencil_host.so`` (the host-side Pencil API:
+``.pecl`` parser, synthetic scenes, tone mapping) for tests and benchmarks.
+
+There is no fallback: if the built libraries are missing, importing a binding
+raises.  Build with ``python -c "import __graft_entry__ as g; g.build()"``
+(or ``make -C pencil_amd``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(_HERE, "libpencil_hip.so")
+HOST_LIB = os.path.join(_HERE, "libpencil_host.so")
+CLI = os.path.join(_HERE, "bin", "pencil")
+
+PCL_OK = 0
+PRIM_SPHERE, PRIM_TRIANGLE, PRIM_CUBE, PRIM_PLANE, PRIM_CYLINDER, PRIM_CONE = range(6)
+PRIM_NAMES = ["sphere", "triangle", "cube", "plane", "cylinder", "cone"]
+MAT_DIFFUSE, MAT_METAL, MAT_DIELECTRIC, MAT_PRINCIPLED = range(4)
+LIGHT_POINT, LIGHT_DIRECTIONAL = range(2)
+
+
+class Prim(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("material_id", C.c_int32), ("p", C.c_float * 12)]
+
+
+class Material(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("albedo", C.c_float * 3), ("reflectivity", C.c_float), ("ior", C.c_float),
+                ("specular", C.c_float), ("shininess", C.c_float), ("specular_color", C.c_float * 3),
+                ("emission", C.c_float * 3)]
+
+
+class Light(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pos_or_dir", C.c_float * 3), ("intensity", C.c_float * 3),
+                ("radius", C.c_float), ("is_emissive", C.c_int32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("prims", C.POINTER(Prim)), ("n_prims", C.c_int32), ("materials", C.POINTER(Material)),
+                ("n_materials", C.c_int32), ("lights", C.POINTER(Light)), ("n_lights", C.c_int32),
+                ("background", C.c_float * 3), ("ambient", C.c_float * 3), ("use_bvh", C.c_int32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("look", C.c_float * 3), ("up", C.c_float * 3), ("vfov_deg", C.c_float),
+                ("aspect", C.c_float), ("aperture", C.c_float), ("focus_dist", C.c_float),
+                ("bokeh_blades", C.c_int32)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("samples", C.c_int32), ("max_depth", C.c_int32),
+                ("shadow_bias", C.c_float), ("soft_shadow_samples", C.c_int32), ("pathtrace", C.c_int32),
+                ("pathtrace_bounces", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("rays_traced", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+class WorkCounts(C.Structure):
+    _fields_ = [("node_visits", C.c_uint64), ("prim_tests", C.c_uint64 * 6)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_prims", C.c_int32), ("n_nodes", C.c_int32), ("depth", C.c_int32), ("root", C.c_int32),
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("upload_ms", C.c_double)]
+
+
+# pcl_ray / pcl_hit as numpy records (32 B / 8 B)
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("dir", "<f4", 3), ("tmin", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim_id", "<i4")])
+
+_P = C.c_void_p
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_hip = None
+_host = None
+
+
+def hip_lib():
+    """The C-ABI library; raises if it was not built."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise RuntimeError(f"{HIP_LIB} is missing: build the HIP core first (make -C pencil_amd)")
+        lib = C.CDLL(HIP_LIB, mode=C.RTLD_GLOBAL)
+        _sig(lib, "pcl_abi_version", C.c_int, [])
+        _sig(lib, "pcl_init", C.c_int, [C.c_int, C.POINTER(_P)])
+        _sig(lib, "pcl_destroy", None, [_P])
+        _sig(lib, "pcl_last_error", C.c_char_p, [_P])
+        _sig(lib, "pcl_scene_upload", C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(C.c_int32)])
+        _sig(lib, "pcl_scene_free", C.c_int, [_P, C.c_int32])
+        _sig(lib, "pcl_scene_info_get", C.c_int, [_P, C.c_int32, C.POINTER(SceneInfo)])
+        _sig(lib, "pcl_scene_leaf_order", C.c_int, [_P, C.c_int32, _P, C.c_int32])
+        _sig(lib, "pcl_bvh_build_host", C.c_int, [C.POINTER(SceneDesc), C.POINTER(SceneInfo), _P, C.c_int32])
+        _sig(lib, "pcl_trace_rays", C.c_int, [_P, C.c_int32, _P, C.c_int32, C.c_int32, _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_trace_rays_device", C.c_int, [_P, C.c_int32, _P, C.c_int32, C.c_int32, _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_trace_rays_counts", C.c_int,
+             [_P, C.c_int32, _P, C.c_int32, C.c_int32, _P, C.POINTER(WorkCounts)])
+        _sig(lib, "pcl_render", C.c_int, [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_render_device", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_render_counts", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, C.POINTER(Stats), C.POINTER(WorkCounts)])
+        _sig(lib, "pcl_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)])
+        _sig(lib, "pcl_device_free", C.c_int, [_P, _P])
+        _sig(lib, "pcl_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t])
+        _sig(lib, "pcl_memcpy_d2h", C.c_int, [_P, _P, _P, C.c_size_t])
+        _hip = lib
+    return _hip
+
+
+def host_lib():
+    """The host-side Pencil API (parser, synthetic scenes, tone map)."""
+    global _host
+    if _host is None:
+        hip_lib()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is missing: build with make -C pencil_amd")
+        lib = C.CDLL(HOST_LIB)
+        _sig(lib, "pclh_load_pecl", _P, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int])
+        _sig(lib, "pclh_make_stress", _P, [C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int])
+        _sig(lib, "pclh_write_stress", C.c_int,
+             [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_char_p, C.c_int])
+        _sig(lib, "pclh_free", None, [_P])
+        _sig(lib, "pclh_desc", C.c_int, [_P, C.POINTER(SceneDesc)])
+        _sig(lib, "pclh_camera", C.c_int, [_P, C.POINTER(Camera)])
+        _sig(lib, "pclh_settings", C.c_int, [_P, C.POINTER(Settings)])
+        _sig(lib, "pclh_unsupported", C.c_int, [_P, C.c_char_p, C.c_int])
+        _sig(lib, "pclh_n_materials", C.c_int, [_P])
+        _sig(lib, "pclh_set_material_color", C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)])
+        _sig(lib, "pclh_tonemap_rgb8", C.c_int, [_P, C.c_int64, C.c_float, _P])
+        _host = lib
+    return _host
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class HostScene:
+    """A scene on the host side of the API: parsed .pecl or synthetic."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise RuntimeError("scene construction failed")
+        self._h = handle
+
+    @classmethod
+    def from_pecl(cls, path: str, width=0, height=0, samples=0, depth=0):
+        err = C.create_string_buffer(4096)
+        h = host_lib().pclh_load_pecl(path.encode(), width, height, samples, depth, err, len(err))
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return cls(h)
+
+    @classmethod
+    def stress(cls, freq=229, n_spheres=4096, seed=1234, width=1920, height=1080):
+        return cls(host_lib().pclh_make_stress(freq, n_spheres, seed, width, height))
+
+    @staticmethod
+    def write_stress(directory: str, name: str, freq=229, n_spheres=4096, seed=1234, width=1920, height=1080):
+        err = C.create_string_buffer(1024)
+        if host_lib().pclh_write_stress(directory.encode(), name.encode(), freq, n_spheres, seed, width, height,
+                                        err, len(err)) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def desc(self) -> SceneDesc:
+        d = SceneDesc()
+        host_lib().pclh_desc(self._h, C.byref(d))
+        return d
+
+    def camera(self) -> Camera:
+        c = Camera()
+        host_lib().pclh_camera(self._h, C.byref(c))
+        return c
+
+    def settings(self) -> Settings:
+        s = Settings()
+        host_lib().pclh_settings(self._h, C.byref(s))
+        return s
+
+    def unsupported(self) -> str:
+        buf = C.create_string_buffer(4096)
+        host_lib().pclh_unsupported(self._h, buf, len(buf))
+        return buf.value.decode()
+
+    def n_materials(self) -> int:
+        return host_lib().pclh_n_materials(self._h)
+
+    def set_material_color(self, m: int, which: int, rgb):
+        arr = (C.c_float * 3)(*[float(x) for x in rgb])
+        if host_lib().pclh_set_material_color(self._h, m, which, arr) != 0:
+            raise ValueError("bad material edit")
+
+    def close(self):
+        if self._h:
+            host_lib().pclh_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tonemap_rgb8(hdr: np.ndarray, exposure: float = 0.0) -> np.ndarray:
+    """Host ACES + gamma 2.2 + 8-bit encode (the CLI's write_image path)."""
+    hdr = np.ascontiguousarray(hdr, dtype=np.float32)
+    out = np.empty(hdr.shape, dtype=np.uint8)
+    host_lib().pclh_tonemap_rgb8(_ptr(hdr), hdr.size // 3, exposure, _ptr(out))
+    return out
+
+
+def bvh_leaf_order_host(desc: SceneDesc):
+    """The core's host BVH build alone (no device): (leaf order, SceneInfo)."""
+    info = SceneInfo()
+    out = np.empty(max(desc.n_prims, 1), dtype=np.int32)
+    rc = hip_lib().pcl_bvh_build_host(C.byref(desc), C.byref(info), _ptr(out), out.size)
+    if rc != PCL_OK:
+        raise RuntimeError(f"pcl_bvh_build_host failed ({rc})")
+    return out[:desc.n_prims], info
+
+
+class CoreError(RuntimeError):
+    pass
+
+
+class Core:
+    """One pcl_ctx: a HIP device and stream."""
+
+    def __init__(self, device: int = 0):
+        self.lib = hip_lib()
+        self.ctx = C.c_void_p()
+        rc = self.lib.pcl_init(device, C.byref(self.ctx))
+        if rc != PCL_OK:
+            raise CoreError(f"pcl_init({device}) failed with {rc}")
+
+    def _check(self, rc, what):
+        if rc != PCL_OK:
+            raise CoreError(f"{what} failed ({rc}): {self.lib.pcl_last_error(self.ctx).decode()}")
+
+    def upload(self, desc: SceneDesc) -> int:
+        sid = C.c_int32(-1)
+        self._check(self.lib.pcl_scene_upload(self.ctx, C.byref(desc), C.byref(sid)), "pcl_scene_upload")
+        return sid.value
+
+    def free_scene(self, sid: int):
+        self._check(self.lib.pcl_scene_free(self.ctx, sid), "pcl_scene_free")
+
+    def scene_info(self, sid: int) -> SceneInfo:
+        info = SceneInfo()
+        self._check(self.lib.pcl_scene_info_get(self.ctx, sid, C.byref(info)), "pcl_scene_info_get")
+        return info
+
+    def leaf_order(self, sid: int) -> np.ndarray:
+        n = self.scene_info(sid).n_prims
+        out = np.empty(max(n, 1), dtype=np.int32)
+        self._check(self.lib.pcl_scene_leaf_order(self.ctx, sid, _ptr(out), out.size), "pcl_scene_leaf_order")
+        return out[:n]
+
+    def trace_rays(self, sid: int, rays: np.ndarray, any_hit: bool = False):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.empty(rays.shape[0], dtype=HIT_DTYPE)
+        st = Stats()
+        self._check(self.lib.pcl_trace_rays(self.ctx, sid, _ptr(rays), rays.shape[0], int(any_hit), _ptr(hits),
+                                            C.byref(st)), "pcl_trace_rays")
+        return hits, st
+
+    def render(self, sid: int, cam: Camera, settings: Settings):
+        img = np.empty((settings.height, settings.width, 3), dtype=np.float32)
+        st = Stats()
+        self._check(self.lib.pcl_render(self.ctx, sid, C.byref(cam), C.byref(settings), _ptr(img), C.byref(st)),
+                    "pcl_render")
+        return img, st
+
+    # device buffers, for timing with inputs resident in HBM
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self.lib.pcl_device_alloc(self.ctx, nbytes, C.byref(p)), "pcl_device_alloc")
+        return p.value
+
+    def free(self, dptr: int):
+        self._check(self.lib.pcl_device_free(self.ctx, C.c_void_p(dptr)), "pcl_device_free")
+
+    def h2d(self, dptr: int, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        self._check(self.lib.pcl_memcpy_h2d(self.ctx, C.c_void_p(dptr), _ptr(arr), arr.nbytes), "pcl_memcpy_h2d")
+
+    def d2h(self, arr: np.ndarray, dptr: int):
+        self._check(self.lib.pcl_memcpy_d2h(self.ctx, _ptr(arr), C.c_void_p(dptr), arr.nbytes), "pcl_memcpy_d2h")
+
+    def trace_rays_device(self, sid: int, d_rays: int, n: int, d_hits: int, any_hit=False) -> Stats:
+        st = Stats()
+        self._check(self.lib.pcl_trace_rays_device(self.ctx, sid, C.c_void_p(d_rays), n, int(any_hit),
+                                                   C.c_void_p(d_hits), C.byref(st)), "pcl_trace_rays_device")
+        return st
+
+    def trace_counts(self, sid: int, d_rays: int, n: int, d_hits: int, any_hit=False) -> WorkCounts:
+        wc = WorkCounts()
+        self._check(self.lib.pcl_trace_rays_counts(self.ctx, sid, C.c_void_p(d_rays), n, int(any_hit),
+                                                   C.c_void_p(d_hits), C.byref(wc)), "pcl_trace_rays_counts")
+        return wc
+
+    def render_device(self, sid: int, cam: Camera, settings: Settings, d_out: int) -> Stats:
+        st = Stats()
+        self._check(self.lib.pcl_render_device(self.ctx, sid, C.byref(cam), C.byref(settings), C.c_void_p(d_out),
+                                               C.byref(st)), "pcl_render_device")
+        return st
+
+    def render_counts(self, sid: int, cam: Camera, settings: Settings, d_out: int):
+        st, wc = Stats(), WorkCounts()
+        self._check(self.lib.pcl_render_counts(self.ctx, sid, C.byref(cam), C.byref(settings), C.c_void_p(d_out),
+                                               C.byref(st), C.byref(wc)), "pcl_render_counts")
+        return st, wc
+
+    def close(self):
+        if self.ctx:
+            self.lib.pcl_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,493 @@
+// core.hip - gfx950 kernels and the C ABI of include/pencil_hip.h.
+//
+// Kernels
+//   k_trace_rays<ANY, COUNT> : one lane per ray, Scene::hit / Scene::occluded
+//                              over the reference-topology BVH.
+//   k_render                 : one lane per pixel (8x8 pixel tile per wave),
+//                              n*n samples in order, Whitted trace() or
+//                              trace_path() with NEE.
+// Both keep the traversal stack's top 16 entries per lane in LDS (16 x 8 B x
+// 128 lanes = 16 KB per block), spilling deeper entries to a private array.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "host_build.hpp"
+#include "pencil_hip.h"
+#include "render.hpp"
+
+using namespace pcl;
+
+namespace {
+
+constexpr int kBlock = 128;
+constexpr int kCtrSlots = 16;   // device counter words (>= CTR_N)
+
+enum { CTR_RAYS = 0, CTR_CLOSEST, CTR_ANY, CTR_NODES, CTR_T0, CTR_N = CTR_T0 + 6 };
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Wave-level sums, one atomic per counter per wave.
+template <bool COUNT>
+__device__ __forceinline__ void flush_counts(const LaneCount<COUNT>& c, unsigned long long* ctr) {
+    constexpr int N = COUNT ? CTR_N : CTR_NODES;
+    uint32_t v[CTR_N] = {c.rays, c.closest, c.anyhit, c.trav.nodes, c.trav.t0, c.trav.t1,
+                         c.trav.t2, c.trav.t3, c.trav.t4, c.trav.t5};
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (v[k]) atomicAdd(ctr + k, (unsigned long long)v[k]);
+    }
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray* __restrict__ rays, int n,
+                                                       pcl_hit* __restrict__ hits, unsigned long long* ctr) {
+    __shared__ uint2 lds[kStackK * kBlock];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    LaneCount<COUNT> c;
+    if (i < n) {
+        const float4* q = reinterpret_cast<const float4*>(rays + i);
+        const float4 a = q[0], b = q[1];
+        Ray r;
+        r.o = f3(a.x, a.y, a.z);
+        r.d = f3(a.w, b.x, b.y);
+        Stack st;
+        st.lds = lds + threadIdx.x;
+        st.stride = kBlock;
+        float tb, ta;
+        const int slot = traverse<ANY>(S, r, b.z, b.w, st, tb, ta, c.trav);
+        pcl_hit h;
+        if (ANY) {
+            h.t = slot >= 0 ? 1.0f : -1.0f;
+            h.prim_id = slot >= 0 ? 1 : -1;
+        } else {
+            h.t = slot >= 0 ? tb : -1.0f;
+            h.prim_id = slot >= 0 ? S.prims[slot].id : -1;
+        }
+        hits[i] = h;
+    }
+    if (COUNT) flush_counts(c, ctr);
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
+                                                   unsigned long long* ctr) {
+    __shared__ uint2 lds[kStackK * kBlock];
+    const int tiles_x = (st.width + 15) / 16;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = tx * 16 + wave * 8 + (lane & 7);
+    const int j = ty * 8 + (lane >> 3);
+    LaneCount<COUNT> c;
+    if (i < st.width && j < st.height) {
+        Frame F[kMaxFrames];
+        Stack stk;
+        stk.lds = lds + threadIdx.x;
+        stk.stride = kBlock;
+        const f3 px = render_pixel(S, C, st, i, j, F, stk, c);
+        float* o = out + ((size_t)j * st.width + i) * 3;
+        o[0] = px.x; o[1] = px.y; o[2] = px.z;
+    }
+    flush_counts(c, ctr);
+}
+
+struct SceneSlot {
+    bool used = false;
+    DevScene dev{};
+    std::vector<void*> bufs;
+    std::vector<int32_t> leaf_order;
+    pcl_scene_info info{};
+};
+
+}  // namespace
+
+struct pcl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* d_ctr = nullptr;
+    std::string err;
+    std::vector<SceneSlot> scenes;
+};
+
+namespace {
+
+int fail(pcl_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                                          \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? PCL_ERR_OOM : PCL_ERR_HIP,               \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+SceneSlot* get_scene(pcl_ctx* ctx, int32_t s) {
+    if (!ctx || s < 0 || s >= (int32_t)ctx->scenes.size() || !ctx->scenes[s].used) return nullptr;
+    return &ctx->scenes[s];
+}
+
+template <class T>
+int upload(pcl_ctx* ctx, SceneSlot& sl, const std::vector<T>& v, const T** out) {
+    *out = nullptr;
+    if (v.empty()) return PCL_OK;
+    void* p = nullptr;
+    HIP_TRY(ctx, hipMalloc(&p, v.size() * sizeof(T)));
+    sl.bufs.push_back(p);
+    HIP_TRY(ctx, hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    *out = static_cast<const T*>(p);
+    sl.info.device_bytes += v.size() * sizeof(T);
+    return PCL_OK;
+}
+
+void release(SceneSlot& sl) {
+    for (void* p : sl.bufs) (void)hipFree(p);
+    sl.bufs.clear();
+    sl.used = false;
+}
+
+int check_settings(pcl_ctx* ctx, const pcl_settings* s) {
+    if (!s || s->width <= 0 || s->height <= 0 || s->samples <= 0)
+        return fail(ctx, PCL_ERR_INVALID, "settings: width, height and samples must be positive");
+    if ((int64_t)s->width * s->height > (int64_t)1 << 28)
+        return fail(ctx, PCL_ERR_LIMIT, "settings: image larger than 2^28 pixels");
+    if (!s->pathtrace && s->max_depth > kMaxFrames)
+        return fail(ctx, PCL_ERR_LIMIT, "settings: max_depth above " + std::to_string(kMaxFrames));
+    if (s->pathtrace && s->pathtrace_bounces > kMaxFrames)
+        return fail(ctx, PCL_ERR_LIMIT, "settings: pathtrace_bounces above " + std::to_string(kMaxFrames));
+    if (s->soft_shadow_samples < 0) return fail(ctx, PCL_ERR_INVALID, "settings: soft_shadow_samples < 0");
+    return PCL_OK;
+}
+
+RSettings to_dev(const pcl_settings* s) {
+    RSettings r;
+    r.width = s->width; r.height = s->height; r.samples = s->samples; r.max_depth = s->max_depth;
+    r.shadow_bias = s->shadow_bias; r.soft_samples = s->soft_shadow_samples;
+    r.pathtrace = s->pathtrace; r.bounces = s->pathtrace_bounces;
+    return r;
+}
+
+void fill_stats(pcl_stats* st, const unsigned long long* h, float kms, double t0) {
+    if (!st) return;
+    st->rays_traced = h[CTR_RAYS];
+    st->closest_queries = h[CTR_CLOSEST];
+    st->anyhit_queries = h[CTR_ANY];
+    st->kernel_ms = kms;
+    st->total_ms = now_ms() - t0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcl_abi_version(void) { return PCL_ABI_VERSION; }
+
+int pcl_init(int device, pcl_ctx** out) {
+    if (!out) return PCL_ERR_INVALID;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PCL_ERR_HIP;
+    if (device < 0 || device >= n) return PCL_ERR_INVALID;
+    auto ctx = std::make_unique<pcl_ctx>();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) return PCL_ERR_HIP;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return PCL_ERR_HIP;
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return PCL_ERR_HIP;
+    if (hipMalloc(&ctx->d_ctr, kCtrSlots * sizeof(unsigned long long)) != hipSuccess) return PCL_ERR_OOM;
+    *out = ctx.release();
+    return PCL_OK;
+}
+
+void pcl_destroy(pcl_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& s : ctx->scenes) if (s.used) release(s);
+    if (ctx->d_ctr) (void)hipFree(ctx->d_ctr);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* pcl_last_error(const pcl_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_out) {
+    if (!ctx || !desc || !scene_out) return fail(ctx, PCL_ERR_INVALID, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const double t0 = now_ms();
+    HostLayout L;
+    std::string err;
+    int rc = build_layout(desc, L, err);
+    if (rc != PCL_OK) return fail(ctx, rc, err);
+    if (L.depth > kStackK + kStackSpill)
+        return fail(ctx, PCL_ERR_LIMIT, "BVH depth " + std::to_string(L.depth) + " exceeds the traversal stack (" +
+                                            std::to_string(kStackK + kStackSpill) + ")");
+    const double t1 = now_ms();
+    int32_t idx = -1;
+    for (size_t i = 0; i < ctx->scenes.size(); ++i)
+        if (!ctx->scenes[i].used) { idx = (int32_t)i; break; }
+    if (idx < 0) { idx = (int32_t)ctx->scenes.size(); ctx->scenes.emplace_back(); }
+    SceneSlot& sl = ctx->scenes[idx];
+    sl = SceneSlot{};
+    sl.used = true;
+    DevScene& D = sl.dev;
+    int r = PCL_OK;
+    if ((r = upload(ctx, sl, L.nodes, &D.nodes)) || (r = upload(ctx, sl, L.prims, &D.prims)) ||
+        (r = upload(ctx, sl, L.mats, &D.mats)) || (r = upload(ctx, sl, L.lights, &D.lights)) ||
+        (r = upload(ctx, sl, L.slot_of, &D.slot_of))) {
+        release(sl);
+        return r;
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) { release(sl); return fail(ctx, PCL_ERR_HIP, hipGetErrorString(e)); }
+    D.n_prims = desc->n_prims;
+    D.n_lights = desc->n_lights;
+    D.root = L.root;
+    D.use_bvh = desc->use_bvh ? 1 : 0;
+    for (int k = 0; k < 6; ++k) D.root_box[k] = L.root_box[k];
+    for (int k = 0; k < 3; ++k) { D.bg[k] = desc->background[k]; D.amb[k] = desc->ambient[k]; }
+    sl.leaf_order = std::move(L.leaf_order);
+    sl.info.n_prims = desc->n_prims;
+    sl.info.n_nodes = (int32_t)L.nodes.size();
+    sl.info.depth = L.depth;
+    sl.info.root = L.root;
+    sl.info.build_ms = t1 - t0;
+    sl.info.upload_ms = now_ms() - t1;
+    *scene_out = idx;
+    return PCL_OK;
+}
+
+int pcl_scene_free(pcl_ctx* ctx, int32_t scene) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    release(*sl);
+    return PCL_OK;
+}
+
+int pcl_scene_info_get(pcl_ctx* ctx, int32_t scene, pcl_scene_info* info) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl || !info) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    *info = sl->info;
+    return PCL_OK;
+}
+
+int pcl_scene_leaf_order(pcl_ctx* ctx, int32_t scene, int32_t* out, int32_t n) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl || !out) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (n < (int32_t)sl->leaf_order.size()) return fail(ctx, PCL_ERR_INVALID, "output too small");
+    if (!sl->leaf_order.empty()) std::memcpy(out, sl->leaf_order.data(), sl->leaf_order.size() * sizeof(int32_t));
+    return PCL_OK;
+}
+
+int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t* leaf_order, int32_t n) {
+    if (!desc) return PCL_ERR_INVALID;
+    const double t0 = now_ms();
+    HostLayout L;
+    std::string err;
+    int rc = build_layout(desc, L, err);
+    if (rc != PCL_OK) return rc;
+    if (info) {
+        *info = pcl_scene_info{};
+        info->n_prims = desc->n_prims;
+        info->n_nodes = (int32_t)L.nodes.size();
+        info->depth = L.depth;
+        info->root = L.root;
+        info->device_bytes = L.nodes.size() * sizeof(NodeRec) + L.prims.size() * sizeof(PrimRec) +
+                             L.mats.size() * sizeof(MatRec) + L.lights.size() * sizeof(LightRec) +
+                             L.slot_of.size() * sizeof(int32_t);
+        info->build_ms = now_ms() - t0;
+    }
+    if (leaf_order) {
+        if (n < (int32_t)L.leaf_order.size()) return PCL_ERR_INVALID;
+        if (!L.leaf_order.empty()) std::memcpy(leaf_order, L.leaf_order.data(), L.leaf_order.size() * sizeof(int32_t));
+    }
+    return PCL_OK;
+}
+
+static int launch_trace(pcl_ctx* ctx, SceneSlot* sl, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
+                        pcl_hit* d_hits, bool count) {
+    const int grid = (n + kBlock - 1) / kBlock;
+    if (any_hit) {
+        if (count) hipLaunchKernelGGL((k_trace_rays<true, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sl->dev, d_rays, n, d_hits, ctx->d_ctr);
+        else hipLaunchKernelGGL((k_trace_rays<true, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sl->dev, d_rays, n, d_hits, ctx->d_ctr);
+    } else {
+        if (count) hipLaunchKernelGGL((k_trace_rays<false, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sl->dev, d_rays, n, d_hits, ctx->d_ctr);
+        else hipLaunchKernelGGL((k_trace_rays<false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sl->dev, d_rays, n, d_hits, ctx->d_ctr);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return PCL_OK;
+}
+
+int pcl_trace_rays_device(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
+                          pcl_hit* d_hits, pcl_stats* stats) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (n < 0 || (n > 0 && (!d_rays || !d_hits))) return fail(ctx, PCL_ERR_INVALID, "bad ray buffers");
+    const double t0 = now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    unsigned long long h[kCtrSlots] = {0};
+    float kms = 0.0f;
+    if (n > 0) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(h), ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        int rc = launch_trace(ctx, sl, d_rays, n, any_hit, d_hits, false);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        HIP_TRY(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+    }
+    if (any_hit) h[CTR_ANY] = (unsigned long long)n; else h[CTR_CLOSEST] = (unsigned long long)n;
+    fill_stats(stats, h, kms, t0);
+    return PCL_OK;
+}
+
+int pcl_trace_rays(pcl_ctx* ctx, int32_t scene, const pcl_ray* rays, int32_t n, int32_t any_hit, pcl_hit* hits,
+                   pcl_stats* stats) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (n < 0 || (n > 0 && (!rays || !hits))) return fail(ctx, PCL_ERR_INVALID, "bad ray buffers");
+    if (n == 0) return pcl_trace_rays_device(ctx, scene, nullptr, 0, any_hit, nullptr, stats);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    void *dr = nullptr, *dh = nullptr;
+    HIP_TRY(ctx, hipMalloc(&dr, (size_t)n * sizeof(pcl_ray)));
+    if (hipMalloc(&dh, (size_t)n * sizeof(pcl_hit)) != hipSuccess) { (void)hipFree(dr); return fail(ctx, PCL_ERR_OOM, "hits"); }
+    int rc = PCL_OK;
+    if (hipMemcpy(dr, rays, (size_t)n * sizeof(pcl_ray), hipMemcpyHostToDevice) != hipSuccess) rc = fail(ctx, PCL_ERR_HIP, "copy rays");
+    if (!rc) rc = pcl_trace_rays_device(ctx, scene, (const pcl_ray*)dr, n, any_hit, (pcl_hit*)dh, stats);
+    if (!rc && hipMemcpy(hits, dh, (size_t)n * sizeof(pcl_hit), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(ctx, PCL_ERR_HIP, "copy hits");
+    (void)hipFree(dr);
+    (void)hipFree(dh);
+    return rc;
+}
+
+// Work counters for a ray batch: internal nodes entered and primitive tests,
+// summed over the batch (a separate instrumented launch).
+static void fill_counts(pcl_work_counts* out, const unsigned long long* h) {
+    if (!out) return;
+    out->node_visits = h[CTR_NODES];
+    for (int k = 0; k < 6; ++k) out->prim_tests[k] = h[CTR_T0 + k];
+}
+
+int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
+                          pcl_hit* d_hits, pcl_work_counts* counts) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (n <= 0 || !d_rays || !d_hits) return fail(ctx, PCL_ERR_INVALID, "bad ray buffers");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    unsigned long long h[kCtrSlots] = {0};
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(h), ctx->stream));
+    int rc = launch_trace(ctx, sl, d_rays, n, any_hit, d_hits, true);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    fill_counts(counts, h);
+    return PCL_OK;
+}
+
+static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                       float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (!cam || !d_out_rgb) return fail(ctx, PCL_ERR_INVALID, "null camera or output");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    const double t0 = now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const CamRec C = make_camera(*cam);
+    const RSettings st = to_dev(settings);
+    const int tiles = ((st.width + 15) / 16) * ((st.height + 7) / 8);
+    unsigned long long h[kCtrSlots] = {0};
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(h), ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (counts)
+        hipLaunchKernelGGL(k_render<true>, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
+    else
+        hipLaunchKernelGGL(k_render<false>, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float kms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+    fill_stats(stats, h, kms, t0);
+    fill_counts(counts, h);
+    return PCL_OK;
+}
+
+int pcl_render_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                      float* d_out_rgb, pcl_stats* stats) {
+    return render_impl(ctx, scene, cam, settings, d_out_rgb, stats, nullptr);
+}
+
+int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                      float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) {
+    pcl_work_counts tmp;
+    return render_impl(ctx, scene, cam, settings, d_out_rgb, stats, counts ? counts : &tmp);
+}
+
+int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings, float* out_rgb,
+               pcl_stats* stats) {
+    if (!out_rgb) return fail(ctx, PCL_ERR_INVALID, "null output");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!get_scene(ctx, scene)) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)settings->width * settings->height * 3 * sizeof(float);
+    void* d = nullptr;
+    HIP_TRY(ctx, hipMalloc(&d, bytes));
+    const double t0 = now_ms();
+    rc = pcl_render_device(ctx, scene, cam, settings, (float*)d, stats);
+    if (!rc && hipMemcpy(out_rgb, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(ctx, PCL_ERR_HIP, "copy image");
+    (void)hipFree(d);
+    if (!rc && stats) stats->total_ms = now_ms() - t0;
+    return rc;
+}
+
+int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr) {
+    if (!ctx || !d_ptr) return fail(ctx, PCL_ERR_INVALID, "null argument");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMalloc(d_ptr, bytes));
+    return PCL_OK;
+}
+int pcl_device_free(pcl_ctx* ctx, void* d_ptr) {
+    if (!ctx) return PCL_ERR_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipFree(d_ptr));
+    return PCL_OK;
+}
+int pcl_memcpy_h2d(pcl_ctx* ctx, void* d_dst, const void* h_src, size_t bytes) {
+    if (!ctx) return PCL_ERR_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+    return PCL_OK;
+}
+int pcl_memcpy_d2h(pcl_ctx* ctx, void* h_dst, const void* d_src, size_t bytes) {
+    if (!ctx) return PCL_ERR_INVALID;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+    return PCL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// host_build.hpp - host side of scene upload: reference-topology BVH build,
+// primitive/material/light flattening and the camera frame.
+#pragma once
+
+#include "pencil_hip.h"
+#include "scene_layout.hpp"
+
+#include <string>
+#include <vector>
+
+namespace pcl {
+
+struct HostLayout {
+    std::vector<NodeRec> nodes;
+    std::vector<PrimRec> prims;       // leaf order
+    std::vector<int32_t> leaf_order;  // slot -> insertion id
+    std::vector<int32_t> slot_of;     // insertion id -> slot (linear scan order)
+    std::vector<MatRec> mats;
+    std::vector<LightRec> lights;
+    int32_t root = kEmptyRoot;
+    float root_box[6] = {0, 0, 0, 0, 0, 0};
+    int32_t depth = 0;
+};
+
+// Returns PCL_OK or an error code with a message in err.
+int build_layout(const pcl_scene_desc* desc, HostLayout& out, std::string& err);
+
+// Camera frame of src/camera.hpp:29-86 (focus_dist <= 0 -> |eye-look|, as
+// ParsedScene::finalize does, src/pecl_parser.hpp:136-140).
+CamRec make_camera(const pcl_camera& c);
+
+}  // namespace pcl

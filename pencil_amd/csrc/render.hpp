@@ -1,0 +1,357 @@
+// render.hpp - camera, Whitted trace() and trace_path() on the device.
+//
+// One lane owns one pixel: the reference shares one xorshift stream across
+// all n*n samples of a pixel and the number of draws per sample depends on
+// the path (src/renderer.hpp:313), so a pixel's samples run in order in its
+// lane.  Whitted recursion (metal: one child, glass: reflect then refract) is
+// an explicit frame stack that combines child results exactly the way the
+// recursive return expressions do, so HDR values are the reference's own.
+#pragma once
+
+#include "intersect.hpp"
+
+namespace pcl {
+
+struct RSettings {
+    int32_t width, height, samples, max_depth;
+    float shadow_bias;
+    int32_t soft_samples, pathtrace, bounces;
+};
+
+// Per-lane work counters; COUNT adds the per-node / per-test counts.
+template <bool COUNT>
+struct LaneCount {
+    uint32_t rays = 0;      // reference convention: trace()/trace_path() calls
+    uint32_t closest = 0;   // closest-hit queries
+    uint32_t anyhit = 0;    // any-hit (shadow) queries
+    TravCount<COUNT> trav;
+};
+
+constexpr int kStackK = 16;       // LDS entries per lane
+constexpr int kStackSpill = 96;   // private overflow entries per lane
+constexpr int kMaxFrames = 16;    // Whitted recursion depth the frame stack holds
+
+using Stack = LaneStack<kStackK, kStackSpill>;
+
+// Scene::hit with the full record of the winner.
+template <class LC>
+PCL_D bool scene_hit(const DevScene& S, const Ray& r, float t_min, float t_max, HitRec& rec, Stack& st, LC& c) {
+    c.closest++;
+    float tb, ta;
+    int slot = traverse<false>(S, r, t_min, t_max, st, tb, ta, c.trav);
+    if (slot < 0) return false;
+    PrimView P = load_prim(S.prims, slot);
+    float t;
+    hit_prim<true>(P, r, t_min, ta, t, &rec);
+    return true;
+}
+
+template <class LC>
+PCL_D bool scene_occluded(const DevScene& S, const Ray& r, float t_min, float t_max, Stack& st, LC& c) {
+    c.anyhit++;
+    float tb, ta;
+    return traverse<true>(S, r, t_min, t_max, st, tb, ta, c.trav) >= 0;
+}
+
+PCL_D f3 ld3(const float* p) { return f3(p[0], p[1], p[2]); }
+
+// Camera::ray_at with DOF (camera.cpp:14-68)
+PCL_D Ray camera_ray(const CamRec& C, float u, float v, Rng& rng) {
+    const f3 ll = ld3(C.lower_left), hz = ld3(C.horizontal), vt = ld3(C.vertical), org = ld3(C.origin);
+    f3 dir = ll + hz * u + vt * v - org;
+    f3 o = org;
+    if (C.aperture > 0.0f) {
+        float fd = C.focus_dist;
+        const f3 forward = (ll + hz * 0.5f + vt * 0.5f - org).normalized();
+        const float denom = dir.normalized().dot(forward);
+        const float focus_t = (fabsf(denom) > 1e-6f) ? fd / denom : fd;
+        f3 focal = org + dir.normalized() * focus_t;
+        float dx, dy;
+        if (C.blades <= 2) {
+            rng.unit_disc(dx, dy);
+        } else {   // polygonal bokeh by rejection, camera.cpp:114-146
+            bool inside = false;
+            for (int attempt = 0; attempt < 16 && !inside; ++attempt) {
+                rng.unit_disc(dx, dy);
+                float angle_step = 2.0f * 3.14159265358979f / C.blades;
+                float rr = sqrtf(dx * dx + dy * dy);
+                float theta = atan2f_cr(dy, dx);
+                if (theta < 0) theta += 2.0f * 3.14159265358979f;
+                int sector = (int)(theta / angle_step);
+                float local_angle = theta - sector * angle_step;
+                float half_step = angle_step * 0.5f;
+                float max_r = cosf_cr(half_step) / cosf_cr(local_angle - half_step);
+                inside = rr <= max_r;
+            }
+            if (!inside) rng.unit_disc(dx, dy);
+        }
+        f3 off = ld3(C.u) * dx + ld3(C.v) * dy;
+        off = off * C.aperture;
+        o = org + off;
+        dir = focal - o;
+    }
+    Ray r;
+    r.o = o; r.d = dir.normalized();
+    return r;
+}
+
+PCL_D void load_mat(const MatRec* M, int id, int& kind, f3& albedo, f3& emission, float& refl, float& ior,
+                    float& spec, float& shin, f3& spec_color) {
+    const MatRec& m = M[id];
+    kind = m.kind;
+    albedo = ld3(m.albedo); emission = ld3(m.emission);
+    refl = m.refl; ior = m.ior; spec = m.spec; shin = m.shin;
+    spec_color = ld3(m.spec_color);
+}
+
+// Disc-jittered point light position (renderer.hpp:524-535 / :891-900)
+PCL_D f3 jitter_light(const f3& lpos, float radius, const f3& p, Rng& rng) {
+    f3 to_light = lpos - p;
+    f3 axis = to_light.normalized();
+    f3 up = (fabsf(axis.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
+    f3 t1 = axis.cross(up).normalized();
+    f3 t2 = axis.cross(t1).normalized();
+    float dx, dy;
+    rng.unit_disc(dx, dy);
+    return lpos + (t1 * dx + t2 * dy) * radius;
+}
+
+// The light loop + ambient of Renderer::trace (renderer.hpp:514-577)
+template <class LC>
+PCL_D f3 whitted_local(const DevScene& S, const RSettings& st, const Ray& r, const HitRec& rec, const f3& albedo,
+                       const f3& emission, float spec, float shin, const f3& spec_color, Rng& rng, Stack& stk,
+                       LC& c) {
+    const f3 sn = rec.normal;
+    f3 local = emission;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const LightRec& L = S.lights[li];
+        const f3 lpos = ld3(L.pos), I = ld3(L.I);
+        f3 contrib(0, 0, 0);
+        const int nsamp = (L.radius > 0.0f) ? st.soft_samples : 1;
+        for (int s = 0; s < nsamp; ++s) {
+            f3 L_pos = lpos;
+            if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) L_pos = jitter_light(lpos, L.radius, rec.p, rng);
+            f3 Ld;
+            float dist = 1.0f;
+            if (L.kind == PCL_LIGHT_POINT) { Ld = L_pos - rec.p; dist = Ld.length(); Ld = Ld / dist; }
+            else { Ld = -lpos; dist = 1e9f; }
+            Ray sr;
+            sr.o = rec.p + rec.normal * st.shadow_bias; sr.d = Ld;
+            if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
+            float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
+            f3 col = albedo * I * ndotl;
+            if (spec > 0.0f && ndotl > 0.0f) {
+                f3 V = -r.d.normalized();
+                f3 H = (Ld + V).normalized();
+                float ndoth = clampf(sn.dot(H), 0.0f, 1.0f);
+                float sp = powf_cr(ndoth, shin);
+                col += spec_color * I * (sp * spec);
+            }
+            contrib += col;
+        }
+        local += contrib / (float)nsamp;
+    }
+    local += albedo * ld3(S.amb);
+    return local;
+}
+
+enum : int { FR_METAL = 0, FR_GLASS_REFL = 1, FR_GLASS_REFR = 2, FR_GLASS_TIR = 3 };
+
+struct Frame {
+    f3 albedo, acc, ro, rd;
+    float w0, w1;
+    int type, depth;
+};
+
+// Pushes the glass frame of trace_dielectric (renderer.hpp:1582-1627) and
+// returns the reflection ray; the refraction ray is kept in the frame.
+PCL_D Ray push_glass(Frame* F, int& nf, const Ray& r, const HitRec& rec, const f3& albedo, float ior, int depth,
+                     float bias) {
+    const bool entering = rec.front;
+    const float safe = fmaxf(ior, 0.01f);
+    const float eta = entering ? (1.0f / safe) : safe;
+    const f3 nf_ = rec.normal;
+    const f3 dn = r.d.normalized();
+    const float cos_i = fabsf(dn.dot(rec.normal));
+    const float sin_t2 = eta * eta * (1.0f - cos_i * cos_i);
+    Ray rr;
+    rr.o = rec.p + rec.normal * bias;
+    rr.d = dn.reflect(rec.normal).normalized();
+    Frame& f = F[nf++];
+    f.albedo = albedo;
+    f.depth = depth;
+    if (sin_t2 > 1.0f) {
+        f.type = FR_GLASS_TIR;
+    } else {
+        const float cos_t = sqrtf(1.0f - sin_t2);
+        f.rd = (dn * eta + nf_ * (eta * cos_i - cos_t)).normalized();
+        f.ro = rec.p - nf_ * bias;
+        const float cs = entering ? cos_i : cos_t;
+        const float kr = schlick(cs, eta);
+        f.w0 = kr;
+        f.w1 = 1.0f - kr;
+        f.type = FR_GLASS_REFL;
+    }
+    return rr;
+}
+
+// Renderer::trace (renderer.hpp:403-822, in-core branches) from ray r with
+// `depth`, continuing below frames F[0..nf) already pushed by the caller
+// (trace_path's glass case).  Returns the value the outermost call returns.
+template <class LC>
+PCL_D f3 whitted(const DevScene& S, const RSettings& st, Ray r, int depth, Rng& rng, Frame* F, int nf, Stack& stk,
+                 LC& c) {
+    f3 result;
+    while (true) {
+        c.rays++;
+        bool descended = false;
+        if (depth <= 0) {
+            result = f3(0, 0, 0);
+        } else {
+            HitRec rec;
+            if (!scene_hit(S, r, 1e-4f, 1e9f, rec, stk, c)) {
+                result = ld3(S.bg);
+            } else {
+                int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+                load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+                f3 local = whitted_local(S, st, r, rec, albedo, emission, spec, shin, spec_color, rng, stk, c);
+                if (kind == PCL_MAT_METAL && refl > 0.0f) {
+                    Frame& f = F[nf++];
+                    f.type = FR_METAL; f.albedo = albedo; f.acc = local; f.w0 = refl; f.depth = depth;
+                    Ray rr;
+                    rr.o = rec.p + rec.normal * st.shadow_bias;
+                    rr.d = r.d.normalized().reflect(rec.normal).normalized();
+                    r = rr; depth = depth - 1; descended = true;
+                } else if (kind == PCL_MAT_DIELECTRIC) {
+                    r = push_glass(F, nf, r, rec, albedo, ior, depth, st.shadow_bias);
+                    depth = depth - 1; descended = true;
+                } else {
+                    result = local;
+                }
+            }
+        }
+        if (descended) continue;
+        // return path: fold child results into their parents
+        bool resumed = false;
+        while (nf > 0) {
+            Frame& f = F[nf - 1];
+            if (f.type == FR_METAL) {
+                // local * (1 - rho) + (incoming * albedo) * rho   (:652-657)
+                result = f.acc * (1.0f - f.w0) + result * f.albedo * f.w0;
+                --nf;
+            } else if (f.type == FR_GLASS_TIR) {
+                result = result * f.albedo;   // :1609-1611
+                --nf;
+            } else if (f.type == FR_GLASS_REFL) {
+                f.acc = result;
+                f.type = FR_GLASS_REFR;
+                r.o = f.ro; r.d = f.rd;
+                depth = f.depth - 1;
+                resumed = true;
+                break;
+            } else {
+                // (refl * kr + refr * kt) * albedo   (:1622-1626)
+                f3 g = f.acc * f.w0 + result * f.w1;
+                result = g * f.albedo;
+                --nf;
+            }
+        }
+        if (!resumed) return result;
+    }
+}
+
+// Renderer::trace_path (renderer.hpp:829-972)
+template <class LC>
+PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, Frame* F, Stack& stk, LC& c) {
+    c.rays++;
+    const int bounces = st.bounces;
+    f3 T(1, 1, 1), R(0, 0, 0);
+    int depth = bounces;
+    while (depth > 0) {
+        HitRec rec;
+        if (!scene_hit(S, cur, 1e-4f, 1e9f, rec, stk, c)) { R += T * ld3(S.bg); break; }
+        int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+        load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+        R += T * emission;
+        const f3 sn = rec.normal;
+        for (int li = 0; li < S.n_lights; ++li) {
+            const LightRec& L = S.lights[li];
+            const f3 lpos = ld3(L.pos);
+            f3 Ld;
+            float dist = 1.0f;
+            if (L.kind == PCL_LIGHT_POINT) {
+                f3 L_pos = lpos;
+                if (L.radius > 0.0f) L_pos = jitter_light(lpos, L.radius, rec.p, rng);
+                Ld = L_pos - rec.p; dist = Ld.length(); Ld = Ld / dist;
+            } else {
+                Ld = -lpos; dist = 1e9f;
+            }
+            Ray sr;
+            sr.o = rec.p + rec.normal * st.shadow_bias; sr.d = Ld;
+            if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
+            float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
+            R += T * albedo * ld3(L.I) * ndotl;
+        }
+        R += T * albedo * ld3(S.amb);
+        if (bounces - depth > 3) {   // Russian roulette, :922-926
+            float p = fmaxf(T.x, fmaxf(T.y, T.z));
+            if (rng.unit() > p) break;
+            T = T / p;
+        }
+        f3 nd;
+        f3 NT = T * albedo;
+        if (kind == PCL_MAT_METAL && refl > 0.0f) {
+            f3 rf = cur.d.normalized().reflect(sn).normalized();
+            f3 up = (fabsf(rf.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
+            f3 t1 = rf.cross(up).normalized();
+            f3 t2 = rf.cross(t1).normalized();
+            float r1 = rng.unit() * 0.1f;
+            float r2 = rng.unit() * 6.28f;
+            nd = (rf + t1 * (r1 * cosf_cr(r2)) + t2 * (r1 * sinf_cr(r2))).normalized();
+            NT = T * albedo * refl;
+        } else if (kind == PCL_MAT_DIELECTRIC) {
+            int nf = 0;
+            Ray rr = push_glass(F, nf, cur, rec, albedo, ior, depth, st.shadow_bias);
+            f3 g = whitted(S, st, rr, depth - 1, rng, F, nf, stk, c);
+            R += T * g - T * emission;
+            break;
+        } else {
+            float r1 = rng.unit() * 6.2831853f;
+            float r2 = rng.unit();
+            float r2s = sqrtf(r2);
+            f3 N = sn;
+            f3 up = (fabsf(N.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
+            f3 Tt = up.cross(N).normalized();
+            f3 B = N.cross(Tt).normalized();
+            nd = (Tt * cosf_cr(r1) * r2s + B * sinf_cr(r1) * r2s + N * sqrtf(1.0f - r2)).normalized();
+        }
+        cur.o = rec.p + rec.normal * st.shadow_bias;
+        cur.d = nd;
+        T = NT;
+        depth--;
+    }
+    return R;
+}
+
+// render_tile_region's per-pixel body (renderer.hpp:313-387, non-adaptive)
+template <class LC>
+PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F, Stack& stk,
+                      LC& c) {
+    const int W = st.width, H = st.height, n = st.samples;
+    Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
+    f3 acc(0, 0, 0);
+    for (int sy = 0; sy < n; ++sy) {
+        for (int sx = 0; sx < n; ++sx) {
+            float u = (i + (sx + 0.5f) / n) / (float)W;
+            float v = 1.0f - (j + (sy + 0.5f) / n) / (float)H;
+            u += rng.sym() * (0.5f / (W * n));
+            v += rng.sym() * (0.5f / (H * n));
+            Ray r = camera_ray(C, u, v, rng);
+            if (st.pathtrace) acc += trace_path(S, st, r, rng, F, stk, c);
+            else acc += whitted(S, st, r, st.max_depth, rng, F, 0, stk, c);
+        }
+    }
+    return acc / (float)(n * n);
+}
+
+}  // namespace pcl

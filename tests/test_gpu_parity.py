@@ -1,0 +1,217 @@
+"""HIP core vs the reference (harness fixtures) and the pinned oracle.
+
+Bars (from the task's parity contract):
+  * primitive ids bit-exact; t within 1e-5 relative (observed: bit-exact);
+  * deterministic Whitted scenes within 1/255 per channel after ACES + gamma;
+    hard-shadow scenes are HDR bit-exact.  Scenes with soft shadows or depth of
+    field use cos/sin: the device evaluates them in double and rounds once,
+    the reference's glibc cosf/sinf differ from that in ~5e-4 of arguments by
+    one ulp, which can flip a penumbra sub-sample; those scenes must be HDR
+    exact on >= 99% of pixels and within 1/255 on >= 99.9%;
+  * path tracing: HDR exact on >= 95% of pixels, RMSE <= 0.01 against the
+    reference at the same spp and RNG streams.
+"""
+import json
+
+import numpy as np
+import pytest
+from PIL import Image
+
+import pencil_amd as pa
+import oracle_lib as ol
+from conftest import SCENES, golden
+from test_oracle import HDR_FIXTURES, render_fixture_inputs
+
+pytestmark = pytest.mark.gpu
+
+TRIG_FREE = {"three_spheres": False, "shapes": False, "cornell_box": False, "hard_shadows": True,
+             "soft_shadows": False, "emissive": False, "dof": False, "glass": False}
+
+
+def load(name, **kw):
+    return pa.HostScene.from_pecl(golden(name + ".pecl"), **kw)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_closest_hit_matches_reference(core, name):
+    hs = load(name)
+    sid = core.upload(hs.desc())
+    try:
+        fx = np.load(golden(f"hits_{name}.npz"))
+        rays = ol.camera_rays(hs.camera(), int(fx["width"]), int(fx["height"]))
+        hits, st = core.trace_rays(sid, rays)
+        assert np.array_equal(hits["prim_id"], fx["prim_id"])
+        assert np.array_equal(hits["t"], fx["t"])
+        assert st.closest_queries == len(rays)
+        assert np.array_equal(core.leaf_order(sid), np.load(golden(f"bvh_{name}.npz"))["leaves"])
+    finally:
+        core.free_scene(sid)
+
+
+def _shadow_rays(hs, osc, W=96, H=72):
+    """Rays from primary hit points toward every light, as trace() casts them."""
+    d = hs.desc()
+    rays = ol.camera_rays(hs.camera(), W, H)
+    rec = osc.hit_records(rays)
+    hit = rec[:, 6] > 0
+    out = []
+    for li in range(d.n_lights):
+        L = d.lights[li]
+        p = rec[hit, 0:3] + rec[hit, 3:6] * np.float32(1e-3)
+        lp = np.array(L.pos_or_dir, dtype=np.float32)
+        v = (lp - rec[hit, 0:3]).astype(np.float32)
+        dist = np.sqrt((v * v).sum(1)).astype(np.float32)
+        r = np.zeros(p.shape[0], dtype=pa.RAY_DTYPE)
+        r["origin"], r["dir"] = p, v / dist[:, None]
+        r["tmin"], r["tmax"] = np.float32(1e-3), dist - np.float32(1e-3)
+        out.append(r)
+    return np.concatenate(out) if out else np.zeros(0, dtype=pa.RAY_DTYPE)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_any_hit_matches_oracle(core, name):
+    hs = load(name)
+    osc = ol.OracleScene(hs.desc())
+    sid = core.upload(hs.desc())
+    try:
+        rays = _shadow_rays(hs, osc)
+        if len(rays) == 0:
+            pytest.skip("no lights")
+        g, _ = core.trace_rays(sid, rays, any_hit=True)
+        o = osc.trace_rays(rays, any_hit=True)
+        assert np.array_equal(g["prim_id"], o["prim_id"])
+    finally:
+        core.free_scene(sid)
+
+
+def test_linear_scan_and_single_primitive(core):
+    hs = load("shapes")
+    d = hs.desc()
+    d.use_bvh = 0
+    osc = ol.OracleScene(d)
+    sid = core.upload(d)
+    try:
+        rays = ol.camera_rays(hs.camera(), 80, 60)
+        g, _ = core.trace_rays(sid, rays)
+        o = osc.trace_rays(rays)
+        assert np.array_equal(g["prim_id"], o["prim_id"]) and np.array_equal(g["t"], o["t"])
+    finally:
+        core.free_scene(sid)
+    d = hs.desc()
+    d.n_prims = 1                       # root is a leaf
+    sid = core.upload(d)
+    try:
+        rays = ol.camera_rays(hs.camera(), 40, 30)
+        g, _ = core.trace_rays(sid, rays)
+        o = ol.OracleScene(d).trace_rays(rays)
+        assert np.array_equal(g["prim_id"], o["prim_id"]) and np.array_equal(g["t"], o["t"])
+        empty, _ = core.trace_rays(sid, np.zeros(0, dtype=pa.RAY_DTYPE))
+        assert empty.shape == (0,)
+    finally:
+        core.free_scene(sid)
+
+
+def test_axis_aligned_and_degenerate_rays(core):
+    hs = load("shapes")
+    osc = ol.OracleScene(hs.desc())
+    sid = core.upload(hs.desc())
+    try:
+        o = np.array([[0, 0.5, -5], [0, 5, 0], [-5, 0.5, 0], [1.5, 0.8, -3], [0, 0.3, 0]] * 4, dtype=np.float32)
+        dirs = np.array([[0, 0, 1], [0, -1, 0], [1, 0, 0], [0, 0, 1], [-0.0, -0.0, 1]] * 4, dtype=np.float32)
+        dirs[5:10] *= -1
+        dirs[10:15] = [[0, 0, 0]] * 5
+        r = np.zeros(len(o), dtype=pa.RAY_DTYPE)
+        r["origin"], r["dir"], r["tmin"], r["tmax"] = o, dirs, 1e-4, 1e9
+        g, _ = core.trace_rays(sid, r)
+        h = osc.trace_rays(r)
+        assert np.array_equal(g["prim_id"], h["prim_id"])
+        assert np.array_equal(g["t"], h["t"])
+    finally:
+        core.free_scene(sid)
+
+
+def test_stress_small_matches_reference(core):
+    fx = np.load(golden("bvh_stress.npz"))
+    p = json.loads(str(fx["params"]))
+    hs = pa.HostScene.stress(**p)
+    sid = core.upload(hs.desc())
+    try:
+        assert np.array_equal(core.leaf_order(sid), fx["leaves"])
+        rays = ol.camera_rays(hs.camera(), p["width"], p["height"])
+        hits, _ = core.trace_rays(sid, rays)
+        assert np.array_equal(hits["prim_id"], fx["hit_prim_id"])
+        assert np.array_equal(hits["t"], fx["hit_t"])
+        img, st = core.render(sid, hs.camera(), hs.settings())
+        assert np.array_equal(img, fx["hdr"])
+        assert st.rays_traced == int(fx["rays"])
+    finally:
+        core.free_scene(sid)
+
+
+def test_stress_full_scene_matches_reference_and_oracle(core):
+    """The 1,052,916-primitive bvh_stress scene at its real size."""
+    fx = np.load(golden("bvh_stress.npz"))
+    p = json.loads(str(fx["full_params"]))
+    hs = pa.HostScene.stress(**p)
+    sid = core.upload(hs.desc())
+    try:
+        info = core.scene_info(sid)
+        assert info.n_prims == int(fx["full_n"])
+        rays = ol.camera_rays(hs.camera(), int(fx["full_hit_width"]), int(fx["full_hit_height"]))
+        hits, _ = core.trace_rays(sid, rays)
+        assert np.array_equal(hits["prim_id"], fx["full_hit_prim_id"])
+        assert np.array_equal(hits["t"], fx["full_hit_t"])
+        # every pixel of the 1080p frame: image and counts vs the oracle
+        img, st = core.render(sid, hs.camera(), hs.settings())
+        osc = ol.OracleScene(hs.desc())
+        oimg, oc = osc.render(hs.camera(), hs.settings())
+        assert np.array_equal(img, oimg)
+        assert st.rays_traced == oc.rays_traced == 1920 * 1080
+        assert st.closest_queries == oc.closest_queries and st.anyhit_queries == oc.anyhit_queries
+    finally:
+        core.free_scene(sid)
+
+
+def _stats(a, b):
+    exact = np.all(a == b, axis=-1).mean()
+    return exact, float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
+
+
+@pytest.mark.parametrize("fx_name", HDR_FIXTURES)
+def test_render_matches_reference_hdr(core, fx_name):
+    fx, hs, s = render_fixture_inputs(fx_name)
+    sid = core.upload(hs.desc())
+    try:
+        img, st = core.render(sid, hs.camera(), s)
+    finally:
+        core.free_scene(sid)
+    ref = fx["hdr"]
+    exact, rmse = _stats(img, ref)
+    d8 = np.abs(pa.tonemap_rgb8(img).astype(int) - pa.tonemap_rgb8(ref).astype(int))
+    within = (d8 <= 1).all(-1).mean()
+    print(f"{fx_name}: exact={exact:.5f} rmse={rmse:.2e} max8={d8.max()} within1={within:.5f} "
+          f"rays={st.rays_traced}/{int(fx['rays'])}")
+    if fx_name.startswith("pt_"):
+        assert exact >= 0.95 and rmse <= 1e-2
+    elif TRIG_FREE.get(fx_name, False):
+        assert exact == 1.0 and st.rays_traced == int(fx["rays"])
+    else:
+        assert exact >= 0.99 and within >= 0.999
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_render_matches_reference_golden_png(core, name):
+    hs = load(name)
+    sid = core.upload(hs.desc())
+    try:
+        img, st = core.render(sid, hs.camera(), hs.settings())
+    finally:
+        core.free_scene(sid)
+    rgb = pa.tonemap_rgb8(img)
+    ref = np.asarray(Image.open(golden(name + ".png")).convert("RGB"))
+    d = np.abs(rgb.astype(np.int16) - ref.astype(np.int16))
+    within = (d <= 1).all(-1).mean()
+    print(f"{name}: max={d.max()} within1={within:.6f} exact={(d == 0).all(-1).mean():.6f}")
+    assert within >= 0.999
+    if TRIG_FREE[name]:
+        assert d.max() == 0
