@@ -89,7 +89,18 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        log("copy probe: 3 x (268435456 B read + 268435456 B written)")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the path would shard pixels or scenes across GPUs, see DESIGN.md)", file=sys.stderr)
@@ -141,6 +152,12 @@ def main():
     core.free_scene(sid)
     core.close()
 
+    traffic = None   # HBM bytes per dispatch from the committed rocprofv3 PMC summary of this kernel
+    tpath = os.path.join(ROOT, "profiles", "k_render_bvh_stress_1080p.traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = round(json.load(f)["hbm_bytes_per_dispatch"])
+
     line = {
         "metric": "Mrays/s at 1080p 1M-tri (plain vs batched); adjoint grad-steps/s at FD tol",
         "value": round(value, 3),
@@ -162,7 +179,7 @@ def main():
                    "kernel_ms": round(kernel_ms, 4), "bvh_build_ms": round(info.build_ms, 1),
                    "path": "plain"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
