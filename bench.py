@@ -89,6 +89,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--plain-only", action="store_true", help="skip the batched wavefront leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")
     args = ap.parse_args()
@@ -141,6 +142,38 @@ def main():
     achieved = bytes_per_launch / (kernel_ms / 1000.0) / 1e9
     log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame wall, {kernel_ms:.3f} ms kernel (HIP events)")
 
+    # batched wavefront tracer on the same frame (autotuned once, untimed)
+    batched = None
+    if not args.plain_only:
+        d_wf = core.alloc(W * H * 3 * 4)
+        job = [(sid, cam, st, d_wf)]
+        best, tune_ms = core.autotune(job)
+        log(f"autotune: slots {best.slots}, blocks/CU {best.blocks_per_cu}, stack {best.stack_k}, "
+            f"iters/graph {best.iters_per_graph} -> {tune_ms:.3f} ms")
+        for _ in range(args.warmup):
+            core.render_batch(job, best)
+        wkms = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            bs = core.render_batch(job, best)
+            wkms.append(bs.kernel_ms)
+        t_end = time.perf_counter()
+        import numpy as np
+        a = np.empty((H, W, 3), dtype=np.float32)
+        b = np.empty((H, W, 3), dtype=np.float32)
+        core.d2h(a, d_out)
+        core.d2h(b, d_wf)
+        rmse = float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
+        wq = bs.closest_queries + bs.anyhit_queries
+        wms = (t_end - t_start) * 1000.0 / args.steps
+        batched = {"ms_per_step": wms, "kernel_ms": sum(wkms) / len(wkms), "queries": wq,
+                   "mrays": wq / (wms / 1000.0) / 1e6, "rmse_vs_plain": rmse, "iterations": bs.iterations,
+                   "config": {"slots": best.slots, "blocks_per_cu": best.blocks_per_cu, "stack_k": best.stack_k,
+                              "iters_per_graph": best.iters_per_graph}}
+        log(f"batched: {wms:.3f} ms/frame wall, {batched['kernel_ms']:.3f} ms on the stream, "
+            f"{batched['mrays']:.1f} Mrays/s, RMSE vs plain {rmse:.3g}")
+        core.free(d_wf)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -177,7 +210,10 @@ def main():
                    "queries_per_frame": queries, "ref_convention_rays_per_frame": cst.rays_traced,
                    "ref_convention_mrays_s": round(cst.rays_traced / (ms_per_step / 1000.0) / 1e6, 3),
                    "kernel_ms": round(kernel_ms, 4), "bvh_build_ms": round(info.build_ms, 1),
-                   "path": "plain"},
+                   "path": "plain",
+                   "plain_mrays_s": round(value, 3),
+                   "batched": ({k: (round(v, 4) if isinstance(v, float) else v) for k, v in batched.items()}
+                               if batched else None)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,

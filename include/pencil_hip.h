@@ -55,7 +55,12 @@ enum {
 
 /* Constructor arguments of one primitive, exactly as the reference's
  * constructor receives them (derived values such as normalized axes are
- * computed by the library the way the constructors do). */
+ * computed by the library the way the constructors do).
+ * OR-ing PCL_PRIM_STORED into `kind` says p[] instead holds the members a
+ * constructed object stores (plane normal / cylinder and cone axis already
+ * unit length; cylinder p[7] still the full height): the library then does
+ * not normalize again, which could move a unit vector by one ulp. */
+#define PCL_PRIM_STORED 0x100
 typedef struct pcl_prim {
     int32_t kind;
     int32_t material_id;
@@ -216,6 +221,47 @@ int pcl_render_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam,
 /* pcl_render_device through the instrumented kernel variant, with counts. */
 int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                       float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts);
+
+/* ---- batched wavefront tracer --------------------------------------------
+ * Renders many (scene, camera, settings) items -- scenes of uneven size and
+ * resolution -- in one job on a persistent wavefront tracer: a fixed pool of
+ * pixel slots runs each pixel's Renderer::render program as a state machine;
+ * closest-hit, shading, any-hit and resolve kernels communicate through
+ * ballot-compacted queues and one iteration is replayed as a hipGraph until
+ * the queues drain.  Results equal pcl_render's for each item.  Replaces a
+ * loop of Renderer::render calls (src/renderer.hpp:198) over the items. */
+typedef struct pcl_batch_item {
+    int32_t      scene;       /* handle from pcl_scene_upload */
+    int32_t      pad;
+    pcl_camera   camera;
+    pcl_settings settings;
+    float*       d_out_rgb;   /* device buffer of width*height*3 floats */
+} pcl_batch_item;
+
+/* Tuning knobs; 0 = library default / autotuned value. */
+typedef struct pcl_wf_config {
+    int32_t slots;            /* pixel slots in flight (queue depth) */
+    int32_t blocks_per_cu;    /* persistent traversal blocks per CU */
+    int32_t stack_k;          /* LDS traversal-stack entries per lane: 8, 16 or 32 */
+    int32_t iters_per_graph;  /* wavefront iterations captured per hipGraph replay */
+} pcl_wf_config;
+
+typedef struct pcl_batch_stats {
+    uint64_t      rays_traced;      /* reference convention, summed over items */
+    uint64_t      closest_queries;
+    uint64_t      anyhit_queries;
+    uint64_t      iterations;       /* wavefront iterations with work */
+    double        kernel_ms;        /* HIP-event time of the whole job on the stream */
+    double        total_ms;
+    pcl_wf_config used;
+} pcl_batch_stats;
+
+int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const pcl_wf_config* cfg,
+                     pcl_batch_stats* stats);
+/* Times candidate configurations on this job (coordinate descent over the
+ * knobs) and returns the fastest; also remembered as the context default. */
+int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best,
+                    double* best_ms);
 
 /* Device-memory helpers for callers without their own allocator. */
 int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr);

@@ -70,6 +70,22 @@ class WorkCounts(C.Structure):
     _fields_ = [("node_visits", C.c_uint64), ("prim_tests", C.c_uint64 * 6)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("pad", C.c_int32), ("camera", Camera), ("settings", Settings),
+                ("d_out_rgb", C.c_void_p)]
+
+
+class WfConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("blocks_per_cu", C.c_int32), ("stack_k", C.c_int32),
+                ("iters_per_graph", C.c_int32)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("rays_traced", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
+                ("iterations", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
+                ("used", WfConfig)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("n_prims", C.c_int32), ("n_nodes", C.c_int32), ("depth", C.c_int32), ("root", C.c_int32),
                 ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("upload_ms", C.c_double)]
@@ -118,6 +134,10 @@ def hip_lib():
              [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, C.POINTER(Stats)])
         _sig(lib, "pcl_render_counts", C.c_int,
              [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, C.POINTER(Stats), C.POINTER(WorkCounts)])
+        _sig(lib, "pcl_render_batch", C.c_int,
+             [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(BatchStats)])
+        _sig(lib, "pcl_wf_autotune", C.c_int,
+             [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(C.c_double)])
         _sig(lib, "pcl_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)])
         _sig(lib, "pcl_device_free", C.c_int, [_P, _P])
         _sig(lib, "pcl_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t])
@@ -330,6 +350,36 @@ class Core:
         self._check(self.lib.pcl_render_counts(self.ctx, sid, C.byref(cam), C.byref(settings), C.c_void_p(d_out),
                                                C.byref(st), C.byref(wc)), "pcl_render_counts")
         return st, wc
+
+    def render_batch(self, jobs, cfg: WfConfig = None):
+        """jobs: list of (scene id, Camera, Settings, device output pointer)."""
+        arr = (BatchItem * max(1, len(jobs)))()
+        for i, (sid, cam, st, dptr) in enumerate(jobs):
+            arr[i].scene, arr[i].camera, arr[i].settings, arr[i].d_out_rgb = sid, cam, st, dptr
+        bs = BatchStats()
+        self._check(self.lib.pcl_render_batch(self.ctx, arr, len(jobs), C.byref(cfg) if cfg else None, C.byref(bs)),
+                    "pcl_render_batch")
+        return bs
+
+    def autotune(self, jobs):
+        arr = (BatchItem * max(1, len(jobs)))()
+        for i, (sid, cam, st, dptr) in enumerate(jobs):
+            arr[i].scene, arr[i].camera, arr[i].settings, arr[i].d_out_rgb = sid, cam, st, dptr
+        best, ms = WfConfig(), C.c_double()
+        self._check(self.lib.pcl_wf_autotune(self.ctx, arr, len(jobs), C.byref(best), C.byref(ms)), "pcl_wf_autotune")
+        return best, ms.value
+
+    def render_wavefront(self, sid: int, cam: Camera, settings: Settings, cfg: WfConfig = None):
+        """One item through the batched tracer, image copied back to the host."""
+        n = settings.width * settings.height * 3
+        d = self.alloc(n * 4)
+        try:
+            bs = self.render_batch([(sid, cam, settings, d)], cfg)
+            img = np.empty((settings.height, settings.width, 3), dtype=np.float32)
+            self.d2h(img, d)
+        finally:
+            self.free(d)
+        return img, bs
 
     def close(self):
         if self.ctx:

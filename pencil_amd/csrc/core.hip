@@ -20,6 +20,7 @@
 #include "host_build.hpp"
 #include "pencil_hip.h"
 #include "render.hpp"
+#include "wavefront.hpp"
 
 using namespace pcl;
 
@@ -108,6 +109,7 @@ struct SceneSlot {
     DevScene dev{};
     std::vector<void*> bufs;
     std::vector<int32_t> leaf_order;
+    std::vector<LightRec> lights;   // host copy, for per-item sample budgets
     pcl_scene_info info{};
 };
 
@@ -115,11 +117,14 @@ struct SceneSlot {
 
 struct pcl_ctx {
     int device = 0;
+    int cus = 256;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* d_ctr = nullptr;
     std::string err;
     std::vector<SceneSlot> scenes;
+    pcl::wf::Engine* wf = nullptr;
+    pcl::wf::Config wf_default;      // set by pcl_wf_autotune
 };
 
 namespace {
@@ -213,6 +218,10 @@ int pcl_init(int device, pcl_ctx** out) {
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return PCL_ERR_HIP;
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return PCL_ERR_HIP;
     if (hipMalloc(&ctx->d_ctr, kCtrSlots * sizeof(unsigned long long)) != hipSuccess) return PCL_ERR_OOM;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->cus = prop.multiProcessorCount;
+    ctx->wf = new pcl::wf::Engine();
     *out = ctx.release();
     return PCL_OK;
 }
@@ -222,6 +231,7 @@ void pcl_destroy(pcl_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& s : ctx->scenes) if (s.used) release(s);
+    delete ctx->wf;
     if (ctx->d_ctr) (void)hipFree(ctx->d_ctr);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -267,6 +277,7 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     for (int k = 0; k < 6; ++k) D.root_box[k] = L.root_box[k];
     for (int k = 0; k < 3; ++k) { D.bg[k] = desc->background[k]; D.amb[k] = desc->ambient[k]; }
     sl.leaf_order = std::move(L.leaf_order);
+    sl.lights = L.lights;
     sl.info.n_prims = desc->n_prims;
     sl.info.n_nodes = (int32_t)L.nodes.size();
     sl.info.depth = L.depth;
@@ -463,6 +474,127 @@ int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_set
     (void)hipFree(d);
     if (!rc && stats) stats->total_ms = now_ms() - t0;
     return rc;
+}
+
+static int batch_prepare(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, std::vector<pcl::wf::ItemRec>& its,
+                         std::vector<DevScene>& scs) {
+    if (!ctx || (n > 0 && !items) || n < 0) return fail(ctx, PCL_ERR_INVALID, "bad batch");
+    its.clear();
+    scs.clear();
+    int64_t begin = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const pcl_batch_item& b = items[i];
+        SceneSlot* sl = get_scene(ctx, b.scene);
+        if (!sl) return fail(ctx, PCL_ERR_INVALID, "batch item " + std::to_string(i) + ": bad scene handle");
+        if (!b.d_out_rgb) return fail(ctx, PCL_ERR_INVALID, "batch item " + std::to_string(i) + ": null output");
+        int rc = check_settings(ctx, &b.settings);
+        if (rc) return rc;
+        if (b.settings.soft_shadow_samples < 1)
+            return fail(ctx, PCL_ERR_INVALID, "wavefront needs soft_shadow_samples >= 1");
+        int budget = 0;
+        for (const auto& L : sl->lights)
+            budget += b.settings.pathtrace ? 1 : ((L.radius > 0.0f) ? b.settings.soft_shadow_samples : 1);
+        if (budget > pcl::wf::kMaxSh)
+            return fail(ctx, PCL_ERR_LIMIT, "batch item " + std::to_string(i) + ": " + std::to_string(budget) +
+                                                " light samples per shading point (max " +
+                                                std::to_string(pcl::wf::kMaxSh) + ")");
+        pcl::wf::ItemRec r{};
+        r.cam = make_camera(b.camera);
+        r.st = to_dev(&b.settings);
+        r.out = b.d_out_rgb;
+        r.scene = i;
+        r.pix_begin = begin;
+        begin += (int64_t)b.settings.width * b.settings.height;
+        its.push_back(r);
+        scs.push_back(sl->dev);
+    }
+    return PCL_OK;
+}
+
+static pcl_wf_config to_abi(const pcl::wf::Config& c) {
+    pcl_wf_config o;
+    o.slots = c.slots; o.blocks_per_cu = c.blocks_per_cu; o.stack_k = c.stack_k; o.iters_per_graph = c.iters_per_graph;
+    return o;
+}
+
+int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const pcl_wf_config* cfg,
+                     pcl_batch_stats* stats) {
+    const double t0 = now_ms();
+    std::vector<pcl::wf::ItemRec> its;
+    std::vector<DevScene> scs;
+    int rc = batch_prepare(ctx, items, n, its, scs);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    pcl::wf::Config c = ctx->wf_default;
+    if (cfg) {
+        if (cfg->slots > 0) c.slots = cfg->slots;
+        if (cfg->blocks_per_cu > 0) c.blocks_per_cu = cfg->blocks_per_cu;
+        if (cfg->stack_k > 0) c.stack_k = cfg->stack_k;
+        if (cfg->iters_per_graph > 0) c.iters_per_graph = cfg->iters_per_graph;
+    }
+    pcl::wf::RunStats rs;
+    std::string err;
+    rc = ctx->wf->run(ctx->stream, its, scs, c, ctx->cus, rs, err);
+    if (rc) return fail(ctx, rc, "wavefront: " + err);
+    if (stats) {
+        stats->rays_traced = rs.rays;
+        stats->closest_queries = rs.closest;
+        stats->anyhit_queries = rs.anyhit;
+        stats->iterations = rs.iterations;
+        stats->kernel_ms = rs.ms;
+        stats->total_ms = now_ms() - t0;
+        stats->used = to_abi(rs.cfg);
+    }
+    return PCL_OK;
+}
+
+int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best, double* best_ms) {
+    std::vector<pcl::wf::ItemRec> its;
+    std::vector<DevScene> scs;
+    int rc = batch_prepare(ctx, items, n, its, scs);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int64_t total = 0;
+    for (auto& it : its) total += (int64_t)it.st.width * it.st.height;
+    if (total <= 0) return fail(ctx, PCL_ERR_INVALID, "empty job");
+    std::string err;
+    auto time_cfg = [&](const pcl::wf::Config& c, float& ms) -> int {
+        pcl::wf::RunStats rs;
+        float bestv = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {   // first run builds the graph
+            int r = ctx->wf->run(ctx->stream, its, scs, c, ctx->cus, rs, err);
+            if (r) return r;
+            if (rep > 0 && rs.ms < bestv) bestv = rs.ms;
+        }
+        ms = bestv;
+        return PCL_OK;
+    };
+    pcl::wf::Config cur;
+    cur.slots = (int)std::min<int64_t>(total, 1 << 20);
+    cur.blocks_per_cu = 8;
+    cur.stack_k = 16;
+    cur.iters_per_graph = 4;
+    float cur_ms = 0;
+    if ((rc = time_cfg(cur, cur_ms))) return fail(ctx, rc, "autotune: " + err);
+    std::vector<int> slot_opts;
+    for (int64_t v : {(int64_t)1 << 18, (int64_t)1 << 19, (int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22})
+        if (v < total) slot_opts.push_back((int)v);
+    slot_opts.push_back((int)std::min<int64_t>(total, 1 << 23));
+    for (int knob = 0; knob < 4; ++knob) {
+        std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 8, 16}
+                              : knob == 2 ? std::vector<int>{8, 16, 32} : std::vector<int>{1, 4, 16};
+        for (int v : opts) {
+            pcl::wf::Config c = cur;
+            (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k : c.iters_per_graph) = v;
+            float ms;
+            if ((rc = time_cfg(c, ms))) return fail(ctx, rc, "autotune: " + err);
+            if (ms < cur_ms) { cur_ms = ms; cur = c; }
+        }
+    }
+    ctx->wf_default = cur;
+    if (best) *best = to_abi(cur);
+    if (best_ms) *best_ms = cur_ms;
+    return PCL_OK;
 }
 
 int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr) {

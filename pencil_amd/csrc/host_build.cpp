@@ -199,11 +199,18 @@ int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
                   " outside [0, " + std::to_string(d->n_materials) + ")";
             return PCL_ERR_INVALID;
         }
+        const int kind = p.kind & 0xFF;
+        const bool stored = (p.kind & PCL_PRIM_STORED) != 0;   // unit vectors already normalized
+        if (p.kind & ~(0xFF | PCL_PRIM_STORED)) {
+            err = "primitive " + std::to_string(i) + " has unknown kind flags";
+            return PCL_ERR_INVALID;
+        }
+        auto unit = [stored](const V& v) { return stored ? v : v.normalized(); };
         PrimRec r{};
-        r.kind = p.kind; r.mat = p.material_id; r.id = i;
+        r.kind = kind; r.mat = p.material_id; r.id = i;
         V p0 = vec(p.p), p1 = vec(p.p + 3), p2 = vec(p.p + 6);
         Box b;
-        switch (p.kind) {
+        switch (kind) {
         case PCL_PRIM_SPHERE:
             put(r.g, p0); r.g[3] = p.p[3];
             b = prim_box(r);
@@ -225,15 +232,15 @@ int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
             break;
         }
         case PCL_PRIM_PLANE:
-            put(r.g, p0); put(r.g + 3, p1.normalized());
+            put(r.g, p0); put(r.g + 3, unit(p1));
             b = prim_box(r);
             break;
         case PCL_PRIM_CYLINDER:
-            put(r.g, p0); put(r.g + 3, p1.normalized()); r.g[6] = p.p[6]; r.g[7] = p.p[7] * 0.5f;
+            put(r.g, p0); put(r.g + 3, unit(p1)); r.g[6] = p.p[6]; r.g[7] = p.p[7] * 0.5f;
             b = prim_box(r);
             break;
         case PCL_PRIM_CONE: {
-            V ax = p1.normalized();
+            V ax = unit(p1);
             put(r.g, p0); put(r.g + 3, ax); r.g[6] = p.p[6]; r.g[7] = p.p[7];
             put(r.g + 8, p0 - ax * p.p[7]);   // Cone ctor, primitives.hpp:511-515
             b = prim_box(r);

@@ -1,0 +1,117 @@
+"""The batched wavefront tracer against the per-pixel megakernel and the reference.
+
+Both paths run the same per-pixel program with the same float operations in
+the same order, so their images must be identical bit for bit; against the
+reference fixtures the same bars as the megakernel hold.
+"""
+import json
+
+import numpy as np
+import pytest
+
+import pencil_amd as pa
+from conftest import SCENES, golden
+from test_oracle import HDR_FIXTURES, render_fixture_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("fx_name", HDR_FIXTURES)
+def test_wavefront_equals_megakernel_and_reference(core, fx_name):
+    fx, hs, s = render_fixture_inputs(fx_name)
+    sid = core.upload(hs.desc())
+    try:
+        mk, mst = core.render(sid, hs.camera(), s)
+        wf, wst = core.render_wavefront(sid, hs.camera(), s)
+    finally:
+        core.free_scene(sid)
+    assert np.array_equal(wf, mk)
+    assert wst.rays_traced == mst.rays_traced
+    assert wst.closest_queries == mst.closest_queries and wst.anyhit_queries == mst.anyhit_queries
+    exact = np.all(wf == fx["hdr"], axis=-1).mean()
+    assert exact >= 0.99
+
+
+@pytest.mark.parametrize("cfg", [dict(slots=1000, blocks_per_cu=1, stack_k=8, iters_per_graph=1),
+                                 dict(slots=4096, blocks_per_cu=16, stack_k=32, iters_per_graph=3)])
+def test_wavefront_knobs_do_not_change_results(core, cfg):
+    hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=64, height=48, samples=2)
+    sid = core.upload(hs.desc())
+    try:
+        mk, _ = core.render(sid, hs.camera(), hs.settings())
+        wf, st = core.render_wavefront(sid, hs.camera(), hs.settings(), pa.WfConfig(**cfg))
+        assert st.used.slots == min(cfg["slots"], 64 * 48) and st.used.stack_k == cfg["stack_k"]
+    finally:
+        core.free_scene(sid)
+    assert np.array_equal(wf, mk)
+
+
+def test_batch_of_uneven_scenes_equals_individual_renders(core):
+    """Several scenes of different size, resolution and mode in one job."""
+    jobs, refs, bufs, sids = [], [], [], []
+    specs = [("three_spheres", dict(width=80, height=60, samples=2), False),
+             ("shapes", dict(width=33, height=17, samples=1), False),
+             ("cornell_box", dict(width=64, height=64, samples=2), True),
+             ("glass", dict(width=50, height=40, samples=1), False)]
+    stress = pa.HostScene.stress(freq=30, n_spheres=200, seed=5, width=96, height=54)
+    try:
+        for name, kw, pt in specs:
+            hs = pa.HostScene.from_pecl(golden(name + ".pecl"), **kw)
+            s = hs.settings()
+            if pt:
+                s.pathtrace, s.pathtrace_bounces = 1, 6
+            sid = core.upload(hs.desc())
+            sids.append(sid)
+            img, _ = core.render(sid, hs.camera(), s)
+            d = core.alloc(s.width * s.height * 12)
+            bufs.append((d, s))
+            jobs.append((sid, hs.camera(), s, d))
+            refs.append(img)
+        sid = core.upload(stress.desc())
+        sids.append(sid)
+        s = stress.settings()
+        img, _ = core.render(sid, stress.camera(), s)
+        d = core.alloc(s.width * s.height * 12)
+        bufs.append((d, s))
+        jobs.append((sid, stress.camera(), s, d))
+        refs.append(img)
+        bs = core.render_batch(jobs)
+        assert bs.iterations > 0
+        for (d, s), ref in zip(bufs, refs):
+            out = np.empty((s.height, s.width, 3), dtype=np.float32)
+            core.d2h(out, d)
+            assert np.array_equal(out, ref)
+    finally:
+        for d, _ in bufs:
+            core.free(d)
+        for sid in sids:
+            core.free_scene(sid)
+
+
+def test_autotune_returns_a_working_config(core):
+    hs = pa.HostScene.from_pecl(golden("shapes.pecl"), width=96, height=72, samples=1)
+    sid = core.upload(hs.desc())
+    d = core.alloc(96 * 72 * 12)
+    try:
+        best, ms = core.autotune([(sid, hs.camera(), hs.settings(), d)])
+        assert best.stack_k in (8, 16, 32) and best.slots > 0 and ms > 0
+        mk, _ = core.render(sid, hs.camera(), hs.settings())
+        wf, _ = core.render_wavefront(sid, hs.camera(), hs.settings(), best)
+        assert np.array_equal(wf, mk)
+    finally:
+        core.free(d)
+        core.free_scene(sid)
+
+
+def test_wavefront_stress_full_frame_matches_megakernel(core):
+    fx = np.load(golden("bvh_stress.npz"))
+    p = json.loads(str(fx["full_params"]))
+    hs = pa.HostScene.stress(**p)
+    sid = core.upload(hs.desc())
+    try:
+        mk, mst = core.render(sid, hs.camera(), hs.settings())
+        wf, wst = core.render_wavefront(sid, hs.camera(), hs.settings())
+        assert np.array_equal(wf, mk)
+        assert (wst.closest_queries, wst.anyhit_queries) == (mst.closest_queries, mst.anyhit_queries)
+    finally:
+        core.free_scene(sid)
