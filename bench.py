@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--plain-only", action="store_true", help="skip the batched wavefront leg")
+    ap.add_argument("--batched-config", default="",
+                    help="slots,blocks_per_cu,stack_k,iters_per_graph for the batched leg instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")
     args = ap.parse_args()
@@ -147,7 +149,11 @@ def main():
     if not args.plain_only:
         d_wf = core.alloc(W * H * 3 * 4)
         job = [(sid, cam, st, d_wf)]
-        best, tune_ms = core.autotune(job)
+        if args.batched_config:
+            best = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")])
+            tune_ms = 0.0
+        else:
+            best, tune_ms = core.autotune(job)
         log(f"autotune: slots {best.slots}, blocks/CU {best.blocks_per_cu}, stack {best.stack_k}, "
             f"iters/graph {best.iters_per_graph} -> {tune_ms:.3f} ms")
         for _ in range(args.warmup):

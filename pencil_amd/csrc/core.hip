@@ -55,7 +55,7 @@ __device__ __forceinline__ void flush_counts(const LaneCount<COUNT>& c, unsigned
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray* __restrict__ rays, int n,
                                                        pcl_hit* __restrict__ hits, unsigned long long* ctr) {
-    __shared__ uint2 lds[kStackK * kBlock];
+    __shared__ unsigned long long lds[kStackK * kBlock];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     LaneCount<COUNT> c;
     if (i < n) {
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
         r.o = f3(a.x, a.y, a.z);
         r.d = f3(a.w, b.x, b.y);
         Stack st;
-        st.lds = lds + threadIdx.x;
+        st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         st.stride = kBlock;
         float tb, ta;
         const int slot = traverse<ANY>(S, r, b.z, b.w, st, tb, ta, c.trav);
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
             h.prim_id = slot >= 0 ? 1 : -1;
         } else {
             h.t = slot >= 0 ? tb : -1.0f;
-            h.prim_id = slot >= 0 ? S.prims[slot].id : -1;
+            h.prim_id = slot >= 0 ? gptr(S.prims)[slot].id : -1;
         }
         hits[i] = h;
     }
@@ -85,7 +85,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
                                                    unsigned long long* ctr) {
-    __shared__ uint2 lds[kStackK * kBlock];
+    __shared__ unsigned long long lds[kStackK * kBlock];
+    // Tiles in row-major block order.  (Giving each of the 8 XCDs a contiguous
+    // band of tiles, for L2 locality, measured 43% slower on the 1M-triangle
+    // frame: the bands' costs differ and the busiest XCD sets the time.)
     const int tiles_x = (st.width + 15) / 16;
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_render(DevScene S, CamRec C, RSettin
     if (i < st.width && j < st.height) {
         Frame F[kMaxFrames];
         Stack stk;
-        stk.lds = lds + threadIdx.x;
+        stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         stk.stride = kBlock;
         const f3 px = render_pixel(S, C, st, i, j, F, stk, c);
         float* o = out + ((size_t)j * st.width + i) * 3;
@@ -504,7 +507,7 @@ static int batch_prepare(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, s
         r.out = b.d_out_rgb;
         r.scene = i;
         r.pix_begin = begin;
-        begin += (int64_t)b.settings.width * b.settings.height;
+        begin += pcl::wf::padded_pixels(b.settings.width, b.settings.height);
         its.push_back(r);
         scs.push_back(sl->dev);
     }
@@ -555,7 +558,7 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     if (rc) return rc;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int64_t total = 0;
-    for (auto& it : its) total += (int64_t)it.st.width * it.st.height;
+    for (auto& it : its) total += pcl::wf::padded_pixels(it.st.width, it.st.height);
     if (total <= 0) return fail(ctx, PCL_ERR_INVALID, "empty job");
     std::string err;
     auto time_cfg = [&](const pcl::wf::Config& c, float& ms) -> int {

@@ -17,6 +17,29 @@
 #define PCL_HD __host__ __device__ __forceinline__
 #define PCL_D __device__ __forceinline__
 
+// Explicit address spaces.  Pointers that reach a kernel through memory (a
+// scene table, a struct) lose the compiler's knowledge that they point to
+// global memory or LDS, and it then emits generic FLAT instructions, which
+// are slower and stall on both memory counters.  Loads from scene data and
+// the LDS traversal stack go through these qualifiers instead.
+#define PCL_GLOBAL __attribute__((address_space(1)))
+#define PCL_LDS __attribute__((address_space(3)))
+
+template <class T>
+__device__ __forceinline__ const PCL_GLOBAL T* gptr(const T* p) { return (const PCL_GLOBAL T*)p; }
+
+typedef float pcl_vf4 __attribute__((ext_vector_type(4)));
+typedef int pcl_vi4 __attribute__((ext_vector_type(4)));
+// 16-byte global loads (one dwordx4) of a 16-byte aligned address.
+__device__ __forceinline__ float4 gload_f4(const void* p) {
+    const pcl_vf4 v = *(const PCL_GLOBAL pcl_vf4*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 gload_i4(const void* p) {
+    const pcl_vi4 v = *(const PCL_GLOBAL pcl_vi4*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 namespace pcl {
 
 struct f3 {

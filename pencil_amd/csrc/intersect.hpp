@@ -38,8 +38,8 @@ struct PrimView {
 PCL_D PrimView load_prim(const PrimRec* P, int slot) {
     const float4* q = reinterpret_cast<const float4*>(P + slot);
     PrimView v;
-    v.a = q[0]; v.b = q[1]; v.c = q[2];
-    int4 d = reinterpret_cast<const int4*>(P + slot)[3];
+    v.a = gload_f4(q); v.b = gload_f4(q + 1); v.c = gload_f4(q + 2);
+    int4 d = gload_i4(q + 3);
     v.kind = d.x; v.mat = d.y;
     return v;
 }
@@ -347,23 +347,23 @@ PCL_D bool slab_test(float lx, float ly, float lz, float hx, float hy, float hz,
 // than K.  Entry = (child ref, entry distance of its box).
 template <int K, int SPILL>
 struct LaneStack {
-    uint2* lds;     // this lane's column base
-    int stride;     // lanes per block
+    PCL_LDS unsigned long long* lds;   // this lane's column base (ref | near << 32)
+    int stride;                        // lanes per block
     int sp;
-    uint2 spill[SPILL];
+    unsigned long long spill[SPILL];
     PCL_D void push(int32_t ref, float near) {
         const int slot = (sp & (K - 1)) * stride;
         if (sp >= K) spill[sp - K] = lds[slot];
-        lds[slot] = make_uint2((uint32_t)ref, __float_as_uint(near));
+        lds[slot] = (unsigned long long)(uint32_t)ref | ((unsigned long long)__float_as_uint(near) << 32);
         ++sp;
     }
     PCL_D int32_t pop(float& near) {
         --sp;
         const int slot = (sp & (K - 1)) * stride;
-        uint2 e = lds[slot];
+        const unsigned long long e = lds[slot];
         if (sp >= K) lds[slot] = spill[sp - K];
-        near = __uint_as_float(e.y);
-        return (int32_t)e.x;
+        near = __uint_as_float((uint32_t)(e >> 32));
+        return (int32_t)(uint32_t)e;
     }
 };
 
@@ -405,7 +405,7 @@ PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, La
     if (!S.use_bvh) {   // linear scan in insertion order, scene.hpp:59-69,77-80
         float closest = t_max;
         for (int i = 0; i < S.n_prims; ++i) {
-            const int slot = S.slot_of[i];
+            const int slot = gptr(S.slot_of)[i];
             PrimView P = load_prim(S.prims, slot);
             float t;
             cnt.test(P.kind);
@@ -428,8 +428,8 @@ PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, La
         if (ref >= 0) {
             cnt.node();
             const float4* np = reinterpret_cast<const float4*>(S.nodes + ref);
-            const float4 a = np[0], b = np[1], c = np[2];
-            const int4 refs = reinterpret_cast<const int4*>(S.nodes + ref)[3];
+            const float4 a = gload_f4(np), b = gload_f4(np + 1), c = gload_f4(np + 2);
+            const int4 refs = gload_i4(np + 3);
             float nl, nr;
             const bool hl = slab_test(a.x, a.y, a.z, a.w, b.x, b.y, s, t_min, t_max, nl);
             const bool hr = slab_test(b.z, b.w, c.x, c.y, c.z, c.w, s, t_min, t_max, nr);

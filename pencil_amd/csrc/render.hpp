@@ -55,6 +55,19 @@ PCL_D bool scene_occluded(const DevScene& S, const Ray& r, float t_min, float t_
 
 PCL_D f3 ld3(const float* p) { return f3(p[0], p[1], p[2]); }
 
+// Field-wise 16-byte global loads of a 48-byte light record.
+PCL_D LightRec load_light(const LightRec* L, int i) {
+    const float4* q = reinterpret_cast<const float4*>(L + i);
+    const float4 a = gload_f4(q), b = gload_f4(q + 1), c = gload_f4(q + 2);
+    LightRec r;
+    r.kind = __float_as_int(a.x);
+    r.pos[0] = a.y; r.pos[1] = a.z; r.pos[2] = a.w;
+    r.I[0] = b.x; r.I[1] = b.y; r.I[2] = b.z;
+    r.radius = b.w;
+    r.emissive = __float_as_int(c.x);
+    return r;
+}
+
 // Camera::ray_at with DOF (camera.cpp:14-68)
 PCL_D Ray camera_ray(const CamRec& C, float u, float v, Rng& rng) {
     const f3 ll = ld3(C.lower_left), hz = ld3(C.horizontal), vt = ld3(C.vertical), org = ld3(C.origin);
@@ -97,11 +110,14 @@ PCL_D Ray camera_ray(const CamRec& C, float u, float v, Rng& rng) {
 
 PCL_D void load_mat(const MatRec* M, int id, int& kind, f3& albedo, f3& emission, float& refl, float& ior,
                     float& spec, float& shin, f3& spec_color) {
-    const MatRec& m = M[id];
-    kind = m.kind;
-    albedo = ld3(m.albedo); emission = ld3(m.emission);
-    refl = m.refl; ior = m.ior; spec = m.spec; shin = m.shin;
-    spec_color = ld3(m.spec_color);
+    // 64-byte record as four 16-byte global loads
+    const float4* q = reinterpret_cast<const float4*>(M + id);
+    const float4 a = gload_f4(q), b = gload_f4(q + 1), c = gload_f4(q + 2), d = gload_f4(q + 3);
+    kind = __float_as_int(a.x);
+    albedo = f3(a.y, a.z, a.w);
+    refl = b.x; ior = b.y; spec = b.z; shin = b.w;
+    spec_color = f3(c.x, c.y, c.z);
+    emission = f3(c.w, d.x, d.y);
 }
 
 // Disc-jittered point light position (renderer.hpp:524-535 / :891-900)
@@ -124,7 +140,7 @@ PCL_D f3 whitted_local(const DevScene& S, const RSettings& st, const Ray& r, con
     const f3 sn = rec.normal;
     f3 local = emission;
     for (int li = 0; li < S.n_lights; ++li) {
-        const LightRec& L = S.lights[li];
+        const LightRec L = load_light(S.lights, li);
         const f3 lpos = ld3(L.pos), I = ld3(L.I);
         f3 contrib(0, 0, 0);
         const int nsamp = (L.radius > 0.0f) ? st.soft_samples : 1;
@@ -275,7 +291,7 @@ PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, F
         R += T * emission;
         const f3 sn = rec.normal;
         for (int li = 0; li < S.n_lights; ++li) {
-            const LightRec& L = S.lights[li];
+            const LightRec L = load_light(S.lights, li);
             const f3 lpos = ld3(L.pos);
             f3 Ld;
             float dist = 1.0f;

@@ -100,6 +100,18 @@ __device__ __forceinline__ int wave_fetch(uint32_t* head) {
     return (int)__shfl(b, 0, kWave);
 }
 
+// Chunk fetch for the persistent loops: one 64-item chunk per wave from a
+// global head.  (Splitting the queue into one contiguous segment per XCD, for
+// L2 locality, measured slower: the segments' costs differ.)
+__device__ __forceinline__ int seg_fetch(uint32_t* heads, int n, int& t, int& hi) {
+    if (t) return -1;
+    hi = n;
+    const int b = wave_fetch(heads);
+    if (b < n) return b;
+    t = 1;
+    return -1;
+}
+
 __device__ __forceinline__ void wave_add_u64(uint32_t v, unsigned long long* dst) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, kWave);
@@ -158,26 +170,35 @@ __device__ void start_sample(const Ctx& C, int s, uint32_t& rays) {
     S.glass_root[s] = 0;
 }
 
-// Pulls the next pixel of the job; false when the job is exhausted.
-__device__ bool next_pixel(const Ctx& C, int s) {
+// Slot program results.
+enum : int { ADV_IDLE = 0, ADV_QUERY = 1, ADV_NEED_PIXEL = 2 };
+
+// Gives slot s job pixel p (render_tile_region's per-pixel setup, :313).
+// Job pixels are numbered in 8x8 tiles (row-major tiles, row-major inside a
+// tile) so that a wave's 64 consecutive slots start on one coherent tile, as
+// in the per-pixel megakernel.  Items are padded to whole tiles; a padding
+// pixel returns false.
+__device__ bool assign_pixel(const Ctx& C, int s, int64_t p) {
     const Slots& S = C.sl;
-    const uint32_t p = atomicAdd(&C.q.ctr[Q_PIXEL], 1u);
-    if ((int64_t)p >= C.q.total_pixels) { S.item[s] = -1; return false; }
-    const int it = find_item(C.items, C.n_items, (int64_t)p);
-    const int pix = (int)((int64_t)p - C.items[it].pix_begin);
-    const int W = C.items[it].st.width;
-    const int i = pix % W, j = pix / W;
+    const int it = find_item(C.items, C.n_items, p);
+    const int r = (int)(p - C.items[it].pix_begin);
+    const int W = C.items[it].st.width, H = C.items[it].st.height;
+    const int tiles_x = (W + 7) >> 3;
+    const int tile = r >> 6, in = r & 63;
+    const int i = (tile % tiles_x) * 8 + (in & 7), j = (tile / tiles_x) * 8 + (in >> 3);
+    if (i >= W || j >= H) return false;
+    const int pix = j * W + i;
     S.item[s] = it;
     S.pixel[s] = pix;
-    S.rng[s] = (uint32_t)(i * 97861u + j * 1973u + 1u);   // renderer.hpp:313
+    S.rng[s] = (uint32_t)(i * 97861u + j * 1973u + 1u);
     S.sample[s] = 0;
     stf(S.acc, s, f3(0, 0, 0));
     return true;
 }
 
-// acc += sample color; next sample, or write the pixel and take a new one.
-// Returns true if the slot has a new sample to trace.
-__device__ bool finish_sample(const Ctx& C, int s, const f3& color, uint32_t& rays) {
+// acc += sample color; starts the next sample (ADV_QUERY pending) or writes
+// the pixel (ADV_NEED_PIXEL).
+__device__ int finish_sample(const Ctx& C, int s, const f3& color, uint32_t& rays) {
     const Slots& S = C.sl;
     f3 acc = ldf(S.acc, s);
     acc += color;
@@ -188,19 +209,17 @@ __device__ bool finish_sample(const Ctx& C, int s, const f3& color, uint32_t& ra
         stf(S.acc, s, acc);
         S.sample[s] = smp;
         start_sample(C, s, rays);
-        return true;
+        return ADV_QUERY;
     }
     const f3 px = acc / (float)(n * n);
     float* o = it.out + (size_t)S.pixel[s] * 3;
     o[0] = px.x; o[1] = px.y; o[2] = px.z;
-    if (!next_pixel(C, s)) return false;
-    start_sample(C, s, rays);
-    return true;
+    return ADV_NEED_PIXEL;
 }
 
 // Folds a Whitted result into the frames (the recursion's return path).
-// Returns 1 if a refraction child must be traced (ray/depth set), 0 if the
-// sample finished (and a new sample may be pending: *again), -1 idle.
+// ADV_QUERY: a refraction child (ray/depth set) or a new sample is pending;
+// ADV_NEED_PIXEL: the pixel is done.
 __device__ int unwind(const Ctx& C, int s, f3 result, uint32_t& rays) {
     const Slots& S = C.sl;
     Frame* F = S.frames + (size_t)s * kMaxFrames;
@@ -220,7 +239,7 @@ __device__ int unwind(const Ctx& C, int s, f3 result, uint32_t& rays) {
             stf(S.ray_o, s, f.ro);
             stf(S.ray_d, s, f.rd);
             S.depth[s] = f.depth - 1;
-            return 1;
+            return ADV_QUERY;
         } else {
             f3 g = f.acc * f.w0 + result * f.w1;
             result = g * f.albedo;
@@ -234,45 +253,72 @@ __device__ int unwind(const Ctx& C, int s, f3 result, uint32_t& rays) {
         R += T * result - T * e;
         result = R;
     }
-    return finish_sample(C, s, result, rays) ? 1 : -1;
+    return finish_sample(C, s, result, rays);
 }
 
-// Drives the slot until it needs a closest-hit query (returns true, ray set)
-// or goes idle.  Whitted nodes with depth <= 0 return black at once.
-__device__ bool advance(const Ctx& C, int s, int status, f3 result, uint32_t& rays) {
+// Drives the slot until it needs a closest-hit query (ADV_QUERY, ray set) or
+// a new pixel (ADV_NEED_PIXEL).  Whitted nodes with depth <= 0 return black
+// at once.  status: ADV_QUERY = the slot's ray is a fresh query; 3 = fold
+// `result` into the frames.
+__device__ int advance(const Ctx& C, int s, int status, f3 result, uint32_t& rays) {
     const Slots& S = C.sl;
-    // status: 1 = slot holds a fresh query to issue, 2 = fold `result`, -1 idle
     for (int guard = 0; guard < 1 << 20; ++guard) {
-        if (status < 0) return false;
-        if (status == 2) { status = unwind(C, s, result, rays); continue; }
-        // a new query for the slot's ray
+        if (status == ADV_NEED_PIXEL || status == ADV_IDLE) return status;
+        if (status == 3) { status = unwind(C, s, result, rays); continue; }
         if (S.mode[s] == M_WHITTED) {
             rays++;                                  // trace() entry (:404)
-            if (S.depth[s] <= 0) { status = 2; result = f3(0, 0, 0); continue; }
-            return true;
+            if (S.depth[s] <= 0) { status = 3; result = f3(0, 0, 0); continue; }
+            return ADV_QUERY;
         }
-        if (S.depth[s] > 0) return true;             // path: while (depth > 0)
-        status = finish_sample(C, s, ldf(S.R, s), rays) ? 1 : -1;
+        if (S.depth[s] > 0) return ADV_QUERY;        // path: while (depth > 0)
+        status = finish_sample(C, s, ldf(S.R, s), rays);
     }
-    return false;
+    return ADV_IDLE;
+}
+
+// Hands new pixels to every lane whose slot finished one: one atomic per wave
+// for the whole wave's requests.  Loops because a new pixel can finish
+// without a query (zero depth / zero bounces).
+__device__ int refill(const Ctx& C, int s, int a, uint32_t& rays) {
+    while (true) {
+        const unsigned long long need = __ballot(a == ADV_NEED_PIXEL);
+        if (!need) return a;
+        const int leader = __ffsll((long long)need) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&C.q.ctr[Q_PIXEL], (uint32_t)__popcll(need));
+        base = __shfl(base, leader, kWave);
+        if (a == ADV_NEED_PIXEL) {
+            const int64_t p = (int64_t)base + __popcll(need & ((1ull << lane_id()) - 1ull));
+            if (p < C.q.total_pixels) {
+                if (assign_pixel(C, s, p)) {
+                    start_sample(C, s, rays);
+                    a = advance(C, s, ADV_QUERY, f3(0, 0, 0), rays);
+                }   // else: a padding pixel, ask again
+            } else {
+                C.sl.item[s] = -1;
+                a = ADV_IDLE;
+            }
+        }
+    }
 }
 
 template <int K>
 __global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
-    __shared__ uint2 lds[K * kWfBlock];
+    __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
-    st.lds = lds + threadIdx.x;
+    st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
     st.stride = kWfBlock;
     const int cur = C.q.ctr[Q_CUR];
     const int n = (int)C.q.ctr[Q_COUNT0 + cur];
     const int32_t* Q = C.q.ray_q[cur];
     const Slots& S = C.sl;
     TravCount<false> cnt;
+    int seg_t = 0, seg_hi = 0;
     while (true) {
-        const int base = wave_fetch(&C.q.ctr[Q_HEAD_CLOSEST]);
-        if (base >= n) break;
+        const int base = seg_fetch(&C.q.ctr[Q_HEAD_CLOSEST], n, seg_t, seg_hi);
+        if (base < 0) break;
         const int k = base + lane_id();
-        if (k < n) {
+        if (k < seg_hi) {
             const int s = Q[k];
             const DevScene& D = C.scenes[C.items[S.item[s]].scene];
             Ray r;
@@ -288,18 +334,19 @@ __global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
 
 template <int K>
 __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
-    __shared__ uint2 lds[K * kWfBlock];
+    __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
-    st.lds = lds + threadIdx.x;
+    st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
     st.stride = kWfBlock;
     const int n = (int)C.q.ctr[Q_SHADOW];
     const Slots& S = C.sl;
     TravCount<false> cnt;
+    int seg_t = 0, seg_hi = 0;
     while (true) {
-        const int base = wave_fetch(&C.q.ctr[Q_HEAD_ANY]);
-        if (base >= n) break;
+        const int base = seg_fetch(&C.q.ctr[Q_HEAD_ANY], n, seg_t, seg_hi);
+        if (base < 0) break;
         const int k = base + lane_id();
-        if (k < n) {
+        if (k < seg_hi) {
             const int2 who = C.q.sh_who[k];
             const float4 a = C.q.sh_ray[2 * k], b = C.q.sh_ray[2 * k + 1];
             const DevScene& D = C.scenes[C.items[S.item[who.x]].scene];
@@ -317,16 +364,17 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // the slot's sample array in the reference's loop order; the shadow rays go
 // straight into the shadow queue at an offset allocated per wave up front
 // (the number of tested samples depends only on the lights and the settings).
-__global__ __launch_bounds__(kWfBlock) void k_shade(Ctx C) {
+__global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
     const int cur = C.q.ctr[Q_CUR];
     const int n = (int)C.q.ctr[Q_COUNT0 + cur];
     const int32_t* Q = C.q.ray_q[cur];
     const Slots& S = C.sl;
+    int seg_t = 0, seg_hi = 0;
     while (true) {
-        const int base = wave_fetch(&C.q.ctr[Q_HEAD_SHADE]);
-        if (base >= n) break;
+        const int base = seg_fetch(&C.q.ctr[Q_HEAD_SHADE], n, seg_t, seg_hi);
+        if (base < 0) break;
         const int k = base + lane_id();
-        const int s = (k < n) ? Q[k] : -1;
+        const int s = (k < seg_hi) ? Q[k] : -1;
         const ItemRec* itp = (s >= 0) ? &C.items[S.item[s]] : nullptr;
         const DevScene* Dp = itp ? &C.scenes[itp->scene] : nullptr;
         const bool path = (s >= 0) && S.mode[s] == M_PATH;
@@ -334,7 +382,7 @@ __global__ __launch_bounds__(kWfBlock) void k_shade(Ctx C) {
         uint32_t n_test = 0;
         if (slot >= 0) {
             for (int li = 0; li < Dp->n_lights; ++li) {
-                const LightRec& L = Dp->lights[li];
+                const LightRec L = load_light(Dp->lights, li);
                 if (!L.emissive) n_test += path ? 1 : ((L.radius > 0.0f) ? itp->st.soft_samples : 1);
             }
         }
@@ -366,7 +414,7 @@ __global__ __launch_bounds__(kWfBlock) void k_shade(Ctx C) {
             const f3 sn = rec.normal;
             const f3 sh_o = rec.p + rec.normal * st.shadow_bias;
             for (int li = 0; li < D.n_lights; ++li) {
-                const LightRec& L = D.lights[li];
+                const LightRec L = load_light(D.lights, li);
                 const f3 lpos = ld3(L.pos), I = ld3(L.I);
                 const int nsamp = path ? 1 : ((L.radius > 0.0f) ? st.soft_samples : 1);
                 for (int q = 0; q < nsamp; ++q) {
@@ -507,19 +555,20 @@ __device__ f3 fold_whitted_local(const Slots& S, int s, const f3& emission, cons
     return local;
 }
 
-__global__ __launch_bounds__(kWfBlock) void k_resolve(Ctx C) {
+__global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
     const int cur = C.q.ctr[Q_CUR];
     const int n = (int)C.q.ctr[Q_COUNT0 + cur];
     const int32_t* Q = C.q.ray_q[cur];
     const Slots& S = C.sl;
     uint32_t rays = 0;
+    int seg_t = 0, seg_hi = 0;
     while (true) {
-        const int base = wave_fetch(&C.q.ctr[Q_HEAD_RESOLVE]);
-        if (base >= n) break;
+        const int base = seg_fetch(&C.q.ctr[Q_HEAD_RESOLVE], n, seg_t, seg_hi);
+        if (base < 0) break;
         const int k = base + lane_id();
-        bool want = false;
+        int a = ADV_IDLE;
         int s = -1;
-        if (k < n) {
+        if (k < seg_hi) {
             s = Q[k];
             const ItemRec& it = C.items[S.item[s]];
             const DevScene& D = C.scenes[it.scene];
@@ -528,8 +577,8 @@ __global__ __launch_bounds__(kWfBlock) void k_resolve(Ctx C) {
             int status;
             f3 result(0, 0, 0);
             switch (outcome) {
-            case V_W_MISS: status = 2; result = ld3(D.bg); break;
-            case V_W_LEAF: status = 2; result = fold_whitted_local(S, s, ldf(S.pend, s), ldf(S.alb, s), amb); break;
+            case V_W_MISS: status = 3; result = ld3(D.bg); break;
+            case V_W_LEAF: status = 3; result = fold_whitted_local(S, s, ldf(S.pend, s), ldf(S.alb, s), amb); break;
             case V_W_CHILD: {
                 if (S.outcome_aux[s]) {
                     S.outcome_aux[s] = 0;          // glass: local discarded (its draws already made)
@@ -538,14 +587,14 @@ __global__ __launch_bounds__(kWfBlock) void k_resolve(Ctx C) {
                     F[S.nf[s] - 1].acc = fold_whitted_local(S, s, ldf(S.pend, s), ldf(S.alb, s), amb);
                 }
                 S.depth[s] = S.depth[s] - 1;
-                status = 1;
+                status = ADV_QUERY;
                 break;
             }
             case V_P_MISS: {
                 f3 R = ldf(S.R, s);
                 R += ldf(S.T, s) * ld3(D.bg);
                 stf(S.R, s, R);
-                status = finish_sample(C, s, R, rays) ? 1 : -1;
+                status = finish_sample(C, s, R, rays);
                 break;
             }
             default: {   // path vertex with a hit: NEE terms in light order, then ambient
@@ -558,46 +607,54 @@ __global__ __launch_bounds__(kWfBlock) void k_resolve(Ctx C) {
                 R += ldf(S.T, s) * ldf(S.alb, s) * amb;
                 stf(S.R, s, R);
                 if (outcome == V_P_END) {
-                    status = finish_sample(C, s, R, rays) ? 1 : -1;
+                    status = finish_sample(C, s, R, rays);
                 } else if (outcome == V_P_GLASS) {
                     S.mode[s] = M_WHITTED;
                     S.glass_root[s] = 1;
                     S.depth[s] = S.depth[s] - 1;   // trace(refl, depth - 1)
-                    status = 1;
+                    status = ADV_QUERY;
                 } else {
                     stf(S.T, s, ldf(S.NT, s));
                     S.depth[s] = S.depth[s] - 1;
-                    status = 1;
+                    status = ADV_QUERY;
                 }
                 break;
             }
             }
-            want = advance(C, s, status, result, rays);
+            a = advance(C, s, status, result, rays);
         }
-        const int pos = wave_append(want, &C.q.ctr[Q_COUNT0 + (cur ^ 1)]);
+        a = refill(C, s, a, rays);
+        const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[Q_COUNT0 + (cur ^ 1)]);
         if (pos >= 0) C.q.ray_q[cur ^ 1][pos] = s;
     }
     wave_add_u64(rays, &C.q.stats[ST_RAYS]);
 }
 
 // Starts the job: every slot takes a pixel and its first query.
-__global__ __launch_bounds__(kWfBlock) void k_start(Ctx C, int n_slots) {
+__global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
     uint32_t rays = 0;
     const int cur = C.q.ctr[Q_CUR];
     for (int base = blockIdx.x * kWfBlock + (threadIdx.x & ~(kWave - 1)); base < n_slots;
          base += gridDim.x * kWfBlock) {
         const int s = base + lane_id();
-        bool want = false;
+        int a = ADV_IDLE;
         if (s < n_slots) {
             C.sl.nf[s] = 0;
             C.sl.glass_root[s] = 0;
             C.sl.outcome_aux[s] = 0;
-            if (next_pixel(C, s)) {
-                start_sample(C, s, rays);
-                want = advance(C, s, 1, f3(0, 0, 0), rays);
+            if ((int64_t)s < C.q.total_pixels) {   // slot s starts on job pixel s
+                if (assign_pixel(C, s, (int64_t)s)) {
+                    start_sample(C, s, rays);
+                    a = advance(C, s, ADV_QUERY, f3(0, 0, 0), rays);
+                } else {
+                    a = ADV_NEED_PIXEL;
+                }
+            } else {
+                C.sl.item[s] = -1;
             }
         }
-        const int pos = wave_append(want, &C.q.ctr[Q_COUNT0 + cur]);
+        a = refill(C, s, a, rays);
+        const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[Q_COUNT0 + cur]);
         if (pos >= 0) C.q.ray_q[cur][pos] = s;
     }
     wave_add_u64(rays, &C.q.stats[ST_RAYS]);
@@ -613,15 +670,12 @@ __global__ void k_swap(Queues q) {
     q.ctr[Q_COUNT0 + cur] = 0;
     q.ctr[Q_CUR] = cur ^ 1;
     q.ctr[Q_SHADOW] = 0;
-    q.ctr[Q_HEAD_CLOSEST] = 0;
-    q.ctr[Q_HEAD_SHADE] = 0;
-    q.ctr[Q_HEAD_ANY] = 0;
-    q.ctr[Q_HEAD_RESOLVE] = 0;
+    for (int i = Q_HEAD_CLOSEST; i < Q_NCTR; ++i) q.ctr[i] = 0;
 }
 
 // Clears the job counters (a kernel, not a memset, so it may sit in a graph).
-__global__ void k_reset(Queues q, uint64_t total_pixels) {
-    if (threadIdx.x < Q_NCTR) q.ctr[threadIdx.x] = 0;
+__global__ void k_reset(Queues q, uint32_t first_free_pixel) {
+    if (threadIdx.x < Q_NCTR) q.ctr[threadIdx.x] = threadIdx.x == Q_PIXEL ? first_free_pixel : 0u;
     if (threadIdx.x < ST_N) q.stats[threadIdx.x] = 0;
 }
 
@@ -704,7 +758,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
                 const Config& cfg_in, int cus, RunStats& out, std::string& err) {
     Config cfg = cfg_in;
     int64_t total = 0;
-    for (const auto& it : items) total += (int64_t)it.st.width * it.st.height;
+    for (const auto& it : items) total += padded_pixels(it.st.width, it.st.height);
     if (cfg.slots <= 0) cfg.slots = (int)std::min<int64_t>(total, 2 << 20);
     cfg.slots = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.slots, total));
     if (cfg.blocks_per_cu <= 0) cfg.blocks_per_cu = 8;
@@ -751,7 +805,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     WF_TRY(hipEventCreate(&e0));
     WF_TRY(hipEventCreate(&e1));
     WF_TRY(hipEventRecord(e0, stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, c.q, (uint64_t)total);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, c.q, (uint32_t)std::min<int64_t>(total, cfg.slots));
     hipLaunchKernelGGL(k_start, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c, cfg.slots);
     WF_TRY(hipGetLastError());
     uint32_t h_ctr[Q_NCTR];

@@ -13,6 +13,9 @@ namespace wf {
 
 constexpr int kMaxSh = 32;   // light samples per shading point (sum over lights)
 
+// Job pixels per item: the image padded to whole 8x8 tiles (tile order).
+inline int64_t padded_pixels(int w, int h) { return (int64_t)((w + 7) / 8 * 8) * (int64_t)((h + 7) / 8 * 8); }
+
 // One job item: a scene, its camera frame, settings and device output.
 struct ItemRec {
     CamRec cam;
@@ -33,9 +36,11 @@ struct Slots {
     Frame* frames;       // [slot][kMaxFrames]
 };
 
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, each with its own L2
 enum : int {
     Q_CUR = 0, Q_COUNT0 = 1, Q_COUNT1 = 2, Q_SHADOW = 3, Q_PIXEL = 4,
-    Q_HEAD_CLOSEST = 5, Q_HEAD_SHADE = 6, Q_HEAD_ANY = 7, Q_HEAD_RESOLVE = 8, Q_NCTR = 16
+    // per-stage work heads, one per XCD segment of the queue
+    Q_HEAD_CLOSEST = 16, Q_HEAD_SHADE = 24, Q_HEAD_ANY = 32, Q_HEAD_RESOLVE = 40, Q_NCTR = 48
 };
 enum : int { ST_RAYS = 0, ST_CLOSEST = 1, ST_ANY = 2, ST_ITERS = 3, ST_N = 8 };
 
