@@ -21,9 +21,11 @@
 //   swap     : flips the queues and clears the work heads
 //
 // Queues are compacted with wave ballots and prefix sums (one atomic per
-// wave).  Kernel grids are fixed (persistent waves fetch 64-item chunks), so
-// an iteration is captured once as a hipGraph and replayed until the ray
-// queue drains.  Slot count, persistent blocks per CU and the LDS stack depth
+// wave) and split into stripes with their own counters (see wavefront.hpp),
+// so no counter takes more than a small share of the atomics.  Kernel grids
+// are fixed (persistent waves fetch 64-item chunks, home stripe first), so an
+// iteration is captured once as a hipGraph and replayed until the ray queue
+// drains.  Slot count, persistent blocks per CU and the LDS stack depth
 // are the autotuned knobs.
 //
 // Floating-point results are the megakernel's (and so the reference's): every
@@ -57,13 +59,17 @@ enum : int {
     V_P_GLASS = 7      // path glass: Whitted subtree from the pushed frame
 };
 
-__device__ __forceinline__ float3 ld(const float* a, int s) { return make_float3(a[3 * s], a[3 * s + 1], a[3 * s + 2]); }
-__device__ __forceinline__ f3 ldf(const float* a, int s) { return f3(a[3 * s], a[3 * s + 1], a[3 * s + 2]); }
-__device__ __forceinline__ void stf(float* a, int s, const f3& v) { a[3 * s] = v.x; a[3 * s + 1] = v.y; a[3 * s + 2] = v.z; }
-
 // ---- wave-level queue helpers --------------------------------------------
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+__device__ __forceinline__ int wave_gid() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
+// A slot's stripe.
+__device__ __forceinline__ int stripe_of(int s) { return (s >> 6) & (kStripes - 1); }
+
+// A counter other waves are changing right now: read it at device scope.
+__device__ __forceinline__ uint32_t ld_live(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Appends one entry per lane with `pred`; returns the lane's index (or -1).
 __device__ __forceinline__ int wave_append(bool pred, uint32_t* counter) {
@@ -93,24 +99,48 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t n, uint32_t* counter)
     return base + incl - n;
 }
 
-// Persistent work fetch: one 64-item chunk per wave.
-__device__ __forceinline__ int wave_fetch(uint32_t* head) {
-    uint32_t b = 0;
-    if (lane_id() == 0) b = atomicAdd(head, (uint32_t)kWave);
-    return (int)__shfl(b, 0, kWave);
-}
+// Striped work fetch.  Stripe g holds len(g) items behind the head counter
+// ctr[sidx(head, g)].  A wave takes `want` items from its home stripe while
+// that lasts; afterwards it reads every stripe's head (one per lane) and takes
+// from the first stripe with work in a wave-specific rotation.  A stripe
+// whose atomic comes back past its end is dead for this wave, so the scan
+// loop ends after at most kStripes misses whatever the reads return.  Returns
+// false once every stripe is exhausted; otherwise items [base, base + got) of
+// stripe g.
+struct Fetcher {
+    unsigned long long dead;   // stripes known to be exhausted
+    int home;                  // home stripe, or -1 once it is exhausted
+    int rot;                   // scan rotation
+    PCL_D explicit Fetcher(int h) : dead(0), home(h), rot((h + 1 + (wave_gid() >> 6) * 7) & (kStripes - 1)) {}
 
-// Chunk fetch for the persistent loops: one 64-item chunk per wave from a
-// global head.  (Splitting the queue into one contiguous segment per XCD, for
-// L2 locality, measured slower: the segments' costs differ.)
-__device__ __forceinline__ int seg_fetch(uint32_t* heads, int n, int& t, int& hi) {
-    if (t) return -1;
-    hi = n;
-    const int b = wave_fetch(heads);
-    if (b < n) return b;
-    t = 1;
-    return -1;
-}
+    template <class Len>
+    PCL_D bool next(uint32_t* ctr, int head, uint32_t want, Len len, int& g, uint32_t& base, uint32_t& got) {
+        if (dead == ~0ull) return false;
+        if (home >= 0) {
+            const uint32_t L = len(home);
+            uint32_t b = 0;
+            if (lane_id() == 0) b = atomicAdd(&ctr[sidx(head, home)], want);
+            b = __shfl(b, 0, kWave);
+            if (b < L) { g = home; base = b; got = min(want, L - b); return true; }
+            dead |= 1ull << home;
+            home = -1;
+        }
+        while (true) {
+            const int mine = lane_id();
+            const bool has = ld_live(&ctr[sidx(head, mine)]) < len(mine);
+            const unsigned long long m = __ballot(has) & ~dead;
+            if (!m) return false;
+            const unsigned long long r = rot ? ((m >> rot) | (m << (kWave - rot))) : m;
+            const int pick = (rot + __ffsll((long long)r) - 1) & (kStripes - 1);
+            const uint32_t L = len(pick);
+            uint32_t b = 0;
+            if (lane_id() == 0) b = atomicAdd(&ctr[sidx(head, pick)], want);
+            b = __shfl(b, 0, kWave);
+            if (b < L) { g = pick; base = b; got = min(want, L - b); return true; }
+            dead |= 1ull << pick;
+        }
+    }
+};
 
 __device__ __forceinline__ void wave_add_u64(uint32_t v, unsigned long long* dst) {
 #pragma unroll
@@ -137,49 +167,79 @@ __device__ __forceinline__ int find_item(const ItemRec* items, int n, int64_t p)
     return lo;
 }
 
-// Camera ray of the slot's current sample (render_tile_region :361-367).
-__device__ void start_sample(const Ctx& C, int s, uint32_t& rays) {
-    const Slots& S = C.sl;
-    const ItemRec& it = C.items[S.item[s]];
+__device__ __forceinline__ float4 f4(const f3& v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ f3 xyz(const float4& v) { return f3(v.x, v.y, v.z); }
+
+// The slot state a kernel works on, held in registers: loaded once, stored
+// once (no read-after-write round trips through global memory).
+struct Lane {
+    int item, pixel, sample;
+    uint32_t rng;
+    int mode, glass_root, outcome_aux, outcome, depth, nf, nsh;
+    f3 o, d, acc, T, R;
+};
+
+__device__ __forceinline__ int pack_flags(const Lane& L) {
+    return L.mode | (L.glass_root << 4) | (L.outcome_aux << 5) | (L.outcome << 8);
+}
+
+__device__ __forceinline__ void load_lane(const Slots& S, int s, Lane& L) {
+    const int4 m = S.meta[s], c = S.ctl[s];
+    L.item = m.x; L.pixel = m.y; L.sample = m.z; L.rng = (uint32_t)m.w;
+    L.mode = c.x & 15; L.glass_root = (c.x >> 4) & 1; L.outcome_aux = (c.x >> 5) & 1; L.outcome = c.x >> 8;
+    L.depth = c.y; L.nf = c.z; L.nsh = c.w;
+    L.o = xyz(S.ray_o[s]); L.d = xyz(S.ray_d[s]);
+    L.acc = xyz(S.acc[s]); L.T = xyz(S.T[s]); L.R = xyz(S.R[s]);
+}
+
+__device__ __forceinline__ void store_lane(const Slots& S, int s, const Lane& L) {
+    S.meta[s] = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
+    S.ctl[s] = make_int4(pack_flags(L), L.depth, L.nf, L.nsh);
+    S.ray_o[s] = f4(L.o);
+    S.ray_d[s] = f4(L.d);
+    S.acc[s] = f4(L.acc);
+    S.T[s] = f4(L.T);
+    S.R[s] = f4(L.R);
+}
+
+// Camera ray of the lane's current sample (render_tile_region :361-367).
+__device__ void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
+    const ItemRec& it = C.items[L.item];
     const RSettings& st = it.st;
     const int W = st.width, H = st.height, n = st.samples;
-    const int pix = S.pixel[s];
-    const int i = pix % W, j = pix / W;
-    const int smp = S.sample[s];
-    const int sy = smp / n, sx = smp % n;
-    Rng rng(S.rng[s]);
+    const int i = L.pixel % W, j = L.pixel / W;
+    const int sy = L.sample / n, sx = L.sample % n;
+    Rng rng(L.rng);
     float u = (i + (sx + 0.5f) / n) / (float)W;
     float v = 1.0f - (j + (sy + 0.5f) / n) / (float)H;
     u += rng.sym() * (0.5f / (W * n));
     v += rng.sym() * (0.5f / (H * n));
     Ray r = camera_ray(it.cam, u, v, rng);
-    S.rng[s] = rng.s;
-    stf(S.ray_o, s, r.o);
-    stf(S.ray_d, s, r.d);
-    S.nf[s] = 0;
+    L.rng = rng.s;
+    L.o = r.o;
+    L.d = r.d;
+    L.nf = 0;
+    L.glass_root = 0;
     if (st.pathtrace) {
         rays++;                      // one per trace_path() call
-        S.mode[s] = M_PATH;
-        S.depth[s] = st.bounces;
-        stf(S.T, s, f3(1, 1, 1));
-        stf(S.R, s, f3(0, 0, 0));
+        L.mode = M_PATH;
+        L.depth = st.bounces;
+        L.T = f3(1, 1, 1);
+        L.R = f3(0, 0, 0);
     } else {
-        S.mode[s] = M_WHITTED;
-        S.depth[s] = st.max_depth;
+        L.mode = M_WHITTED;
+        L.depth = st.max_depth;
     }
-    S.glass_root[s] = 0;
 }
 
 // Slot program results.
 enum : int { ADV_IDLE = 0, ADV_QUERY = 1, ADV_NEED_PIXEL = 2 };
 
-// Gives slot s job pixel p (render_tile_region's per-pixel setup, :313).
 // Job pixels are numbered in 8x8 tiles (row-major tiles, row-major inside a
 // tile) so that a wave's 64 consecutive slots start on one coherent tile, as
 // in the per-pixel megakernel.  Items are padded to whole tiles; a padding
-// pixel returns false.
-__device__ bool assign_pixel(const Ctx& C, int s, int64_t p) {
-    const Slots& S = C.sl;
+// pixel returns false.  Sets up render_tile_region's per-pixel state (:313).
+__device__ bool assign_pixel(const Ctx& C, Lane& L, int64_t p) {
     const int it = find_item(C.items, C.n_items, p);
     const int r = (int)(p - C.items[it].pix_begin);
     const int W = C.items[it].st.width, H = C.items[it].st.height;
@@ -187,32 +247,27 @@ __device__ bool assign_pixel(const Ctx& C, int s, int64_t p) {
     const int tile = r >> 6, in = r & 63;
     const int i = (tile % tiles_x) * 8 + (in & 7), j = (tile / tiles_x) * 8 + (in >> 3);
     if (i >= W || j >= H) return false;
-    const int pix = j * W + i;
-    S.item[s] = it;
-    S.pixel[s] = pix;
-    S.rng[s] = (uint32_t)(i * 97861u + j * 1973u + 1u);
-    S.sample[s] = 0;
-    stf(S.acc, s, f3(0, 0, 0));
+    L.item = it;
+    L.pixel = j * W + i;
+    L.rng = (uint32_t)(i * 97861u + j * 1973u + 1u);
+    L.sample = 0;
+    L.acc = f3(0, 0, 0);
     return true;
 }
 
 // acc += sample color; starts the next sample (ADV_QUERY pending) or writes
 // the pixel (ADV_NEED_PIXEL).
-__device__ int finish_sample(const Ctx& C, int s, const f3& color, uint32_t& rays) {
-    const Slots& S = C.sl;
-    f3 acc = ldf(S.acc, s);
-    acc += color;
-    const ItemRec& it = C.items[S.item[s]];
+__device__ int finish_sample(const Ctx& C, Lane& L, const f3& color, uint32_t& rays) {
+    L.acc += color;
+    const ItemRec& it = C.items[L.item];
     const int n = it.st.samples;
-    int smp = S.sample[s] + 1;
-    if (smp < n * n) {
-        stf(S.acc, s, acc);
-        S.sample[s] = smp;
-        start_sample(C, s, rays);
+    if (L.sample + 1 < n * n) {
+        L.sample += 1;
+        start_sample(C, L, rays);
         return ADV_QUERY;
     }
-    const f3 px = acc / (float)(n * n);
-    float* o = it.out + (size_t)S.pixel[s] * 3;
+    const f3 px = L.acc / (float)(n * n);
+    float* o = it.out + (size_t)L.pixel * 3;
     o[0] = px.x; o[1] = px.y; o[2] = px.z;
     return ADV_NEED_PIXEL;
 }
@@ -220,83 +275,84 @@ __device__ int finish_sample(const Ctx& C, int s, const f3& color, uint32_t& ray
 // Folds a Whitted result into the frames (the recursion's return path).
 // ADV_QUERY: a refraction child (ray/depth set) or a new sample is pending;
 // ADV_NEED_PIXEL: the pixel is done.
-__device__ int unwind(const Ctx& C, int s, f3 result, uint32_t& rays) {
-    const Slots& S = C.sl;
-    Frame* F = S.frames + (size_t)s * kMaxFrames;
-    int nf = S.nf[s];
-    while (nf > 0) {
-        Frame& f = F[nf - 1];
+__device__ int unwind(const Ctx& C, int s, Lane& L, f3 result, uint32_t& rays) {
+    Frame* F = C.sl.frames + (size_t)s * kMaxFrames;
+    while (L.nf > 0) {
+        Frame& f = F[L.nf - 1];
         if (f.type == FR_METAL) {
             result = f.acc * (1.0f - f.w0) + result * f.albedo * f.w0;
-            --nf;
+            --L.nf;
         } else if (f.type == FR_GLASS_TIR) {
             result = result * f.albedo;
-            --nf;
+            --L.nf;
         } else if (f.type == FR_GLASS_REFL) {
             f.acc = result;
             f.type = FR_GLASS_REFR;
-            S.nf[s] = nf;
-            stf(S.ray_o, s, f.ro);
-            stf(S.ray_d, s, f.rd);
-            S.depth[s] = f.depth - 1;
+            L.o = f.ro;
+            L.d = f.rd;
+            L.depth = f.depth - 1;
             return ADV_QUERY;
         } else {
             f3 g = f.acc * f.w0 + result * f.w1;
             result = g * f.albedo;
-            --nf;
+            --L.nf;
         }
     }
-    S.nf[s] = 0;
-    if (S.glass_root[s]) {       // trace_path's glass: R += T*glass - T*emission
-        const f3 T = ldf(S.gT, s), e = ldf(S.gE, s);
-        f3 R = ldf(S.R, s);
-        R += T * result - T * e;
-        result = R;
+    if (L.glass_root) {       // trace_path's glass: R += T*glass - T*emission
+        const f3 T = xyz(C.sl.gT[s]), e = xyz(C.sl.gE[s]);
+        L.R += T * result - T * e;
+        result = L.R;
     }
-    return finish_sample(C, s, result, rays);
+    return finish_sample(C, L, result, rays);
 }
 
-// Drives the slot until it needs a closest-hit query (ADV_QUERY, ray set) or
+// Drives the lane until it needs a closest-hit query (ADV_QUERY, ray set) or
 // a new pixel (ADV_NEED_PIXEL).  Whitted nodes with depth <= 0 return black
-// at once.  status: ADV_QUERY = the slot's ray is a fresh query; 3 = fold
+// at once.  status: ADV_QUERY = the lane's ray is a fresh query; 3 = fold
 // `result` into the frames.
-__device__ int advance(const Ctx& C, int s, int status, f3 result, uint32_t& rays) {
-    const Slots& S = C.sl;
+__device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint32_t& rays) {
     for (int guard = 0; guard < 1 << 20; ++guard) {
         if (status == ADV_NEED_PIXEL || status == ADV_IDLE) return status;
-        if (status == 3) { status = unwind(C, s, result, rays); continue; }
-        if (S.mode[s] == M_WHITTED) {
+        if (status == 3) { status = unwind(C, s, L, result, rays); continue; }
+        if (L.mode == M_WHITTED) {
             rays++;                                  // trace() entry (:404)
-            if (S.depth[s] <= 0) { status = 3; result = f3(0, 0, 0); continue; }
+            if (L.depth <= 0) { status = 3; result = f3(0, 0, 0); continue; }
             return ADV_QUERY;
         }
-        if (S.depth[s] > 0) return ADV_QUERY;        // path: while (depth > 0)
-        status = finish_sample(C, s, ldf(S.R, s), rays);
+        if (L.depth > 0) return ADV_QUERY;           // path: while (depth > 0)
+        status = finish_sample(C, L, L.R, rays);
     }
     return ADV_IDLE;
 }
 
-// Hands new pixels to every lane whose slot finished one: one atomic per wave
-// for the whole wave's requests.  Loops because a new pixel can finish
-// without a query (zero depth / zero bounces).
-__device__ int refill(const Ctx& C, int s, int a, uint32_t& rays) {
+// Hands new pixels to every lane whose slot finished one: one fetch per wave
+// for the whole wave's requests.  The job's 64-pixel tiles are dealt to the
+// stripes round-robin (tile t to stripe t % kStripes), so while the stripes
+// advance together the whole GPU works on one front of the image, whose BVH
+// working set the caches share.  Loops because a new pixel can finish
+// without a query (zero depth / zero bounces) or be a tile's padding.
+__device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t& rays) {
+    const uint32_t tiles = (uint32_t)(C.q.total_pixels >> 6);
+    auto len = [&](int g) -> uint32_t {
+        return (tiles / kStripes + ((uint32_t)g < tiles % kStripes ? 1u : 0u)) * 64u;
+    };
     while (true) {
         const unsigned long long need = __ballot(a == ADV_NEED_PIXEL);
         if (!need) return a;
-        const int leader = __ffsll((long long)need) - 1;
-        uint32_t base = 0;
-        if (lane_id() == leader) base = atomicAdd(&C.q.ctr[Q_PIXEL], (uint32_t)__popcll(need));
-        base = __shfl(base, leader, kWave);
+        int g = 0;
+        uint32_t base = 0, got = 0;
+        const bool ok = pf.next(C.q.ctr, S_PIXEL, (uint32_t)__popcll(need), len, g, base, got);
         if (a == ADV_NEED_PIXEL) {
-            const int64_t p = (int64_t)base + __popcll(need & ((1ull << lane_id()) - 1ull));
-            if (p < C.q.total_pixels) {
-                if (assign_pixel(C, s, p)) {
-                    start_sample(C, s, rays);
-                    a = advance(C, s, ADV_QUERY, f3(0, 0, 0), rays);
-                }   // else: a padding pixel, ask again
-            } else {
-                C.sl.item[s] = -1;
+            if (!ok) {
+                L.item = -1;
                 a = ADV_IDLE;
+            } else {
+                const uint32_t r = (uint32_t)__popcll(need & ((1ull << lane_id()) - 1ull));
+                const uint32_t k = base + r;   // the stripe's k-th pixel
+                if (r < got && assign_pixel(C, L, ((int64_t)(k >> 6) * kStripes + g) * 64 + (k & 63))) {
+                    start_sample(C, L, rays);
+                    a = advance(C, s, L, ADV_QUERY, f3(0, 0, 0), rays);
+                }
             }
         }
     }
@@ -308,26 +364,26 @@ __global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
     LaneStack<K, kStackK + kStackSpill - K> st;
     st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
     st.stride = kWfBlock;
-    const int cur = C.q.ctr[Q_CUR];
-    const int n = (int)C.q.ctr[Q_COUNT0 + cur];
-    const int32_t* Q = C.q.ray_q[cur];
+    const int cur = C.q.ctr[C_CUR];
+    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
     TravCount<false> cnt;
-    int seg_t = 0, seg_hi = 0;
-    while (true) {
-        const int base = seg_fetch(&C.q.ctr[Q_HEAD_CLOSEST], n, seg_t, seg_hi);
-        if (base < 0) break;
-        const int k = base + lane_id();
-        if (k < seg_hi) {
-            const int s = Q[k];
-            const DevScene& D = C.scenes[C.items[S.item[s]].scene];
+    Fetcher f(wave_gid() & (kStripes - 1));
+    int g;
+    uint32_t base, got;
+    while (f.next(C.q.ctr, S_HEAD_CLOSEST, kWave, len, g, base, got)) {
+        if ((uint32_t)lane_id() < got) {
+            const int s = C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()];
+            const DevScene& D = C.scenes[C.items[S.meta[s].x].scene];
+            const float4 o = S.ray_o[s], d = S.ray_d[s];
             Ray r;
-            r.o = ldf(S.ray_o, s);
-            r.d = ldf(S.ray_d, s);
+            r.o = xyz(o);
+            r.d = xyz(d);
             float tb, ta;
             const int slot = traverse<false>(D, r, 1e-4f, 1e9f, st, tb, ta, cnt);
-            S.hit[s] = slot;
-            S.tacc[s] = ta;
+            // the hit rides in the ray's fourth lanes: (o, t_max at acceptance), (d, slot)
+            S.ray_o[s] = make_float4(o.x, o.y, o.z, ta);
+            S.ray_d[s] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
         }
     }
 }
@@ -338,18 +394,18 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
     LaneStack<K, kStackK + kStackSpill - K> st;
     st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
     st.stride = kWfBlock;
-    const int n = (int)C.q.ctr[Q_SHADOW];
+    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_SHADOW, g)]; };
     const Slots& S = C.sl;
     TravCount<false> cnt;
-    int seg_t = 0, seg_hi = 0;
-    while (true) {
-        const int base = seg_fetch(&C.q.ctr[Q_HEAD_ANY], n, seg_t, seg_hi);
-        if (base < 0) break;
-        const int k = base + lane_id();
-        if (k < seg_hi) {
+    Fetcher f(wave_gid() & (kStripes - 1));
+    int g;
+    uint32_t base, got;
+    while (f.next(C.q.ctr, S_HEAD_ANY, kWave, len, g, base, got)) {
+        if ((uint32_t)lane_id() < got) {
+            const size_t k = (size_t)g * C.q.shcap + base + lane_id();
             const int2 who = C.q.sh_who[k];
             const float4 a = C.q.sh_ray[2 * k], b = C.q.sh_ray[2 * k + 1];
-            const DevScene& D = C.scenes[C.items[S.item[who.x]].scene];
+            const DevScene& D = C.scenes[C.items[S.meta[who.x].x].scene];
             Ray r;
             r.o = f3(a.x, a.y, a.z);
             r.d = f3(b.x, b.y, b.z);
@@ -365,20 +421,21 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // straight into the shadow queue at an offset allocated per wave up front
 // (the number of tested samples depends only on the lights and the settings).
 __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
-    const int cur = C.q.ctr[Q_CUR];
-    const int n = (int)C.q.ctr[Q_COUNT0 + cur];
-    const int32_t* Q = C.q.ray_q[cur];
+    const int cur = C.q.ctr[C_CUR];
+    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
-    int seg_t = 0, seg_hi = 0;
-    while (true) {
-        const int base = seg_fetch(&C.q.ctr[Q_HEAD_SHADE], n, seg_t, seg_hi);
-        if (base < 0) break;
-        const int k = base + lane_id();
-        const int s = (k < seg_hi) ? Q[k] : -1;
-        const ItemRec* itp = (s >= 0) ? &C.items[S.item[s]] : nullptr;
+    Fetcher f(wave_gid() & (kStripes - 1));
+    int g;
+    uint32_t base, got;
+    while (f.next(C.q.ctr, S_HEAD_SHADE, kWave, len, g, base, got)) {
+        const int s = ((uint32_t)lane_id() < got) ? C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()] : -1;
+        int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
+        float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
+        if (s >= 0) { meta = S.meta[s]; ctl = S.ctl[s]; ro = S.ray_o[s]; rd = S.ray_d[s]; }
+        const ItemRec* itp = (s >= 0) ? &C.items[meta.x] : nullptr;
         const DevScene* Dp = itp ? &C.scenes[itp->scene] : nullptr;
-        const bool path = (s >= 0) && S.mode[s] == M_PATH;
-        const int slot = (s >= 0) ? S.hit[s] : -1;
+        const bool path = (ctl.x & 15) == M_PATH;
+        const int slot = (s >= 0) ? __float_as_int(rd.w) : -1;
         uint32_t n_test = 0;
         if (slot >= 0) {
             for (int li = 0; li < Dp->n_lights; ++li) {
@@ -386,31 +443,31 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
                 if (!L.emissive) n_test += path ? 1 : ((L.radius > 0.0f) ? itp->st.soft_samples : 1);
             }
         }
-        uint32_t o = wave_append_n(n_test, &C.q.ctr[Q_SHADOW]);
+        size_t o = (size_t)g * C.q.shcap + wave_append_n(n_test, &C.q.ctr[sidx(S_SHADOW, g)]);
         if (s < 0) continue;
         const ItemRec& it = *itp;
         const DevScene& D = *Dp;
         const RSettings& st = it.st;
         Ray r;
-        r.o = ldf(S.ray_o, s);
-        r.d = ldf(S.ray_d, s);
+        r.o = xyz(ro);
+        r.d = xyz(rd);
         int outcome;
-        S.nsh[s] = 0;
+        int flags = ctl.x & 0xF0;        // keeps glass_root / outcome_aux
+        int nf = ctl.z, ns = 0;
         if (slot < 0) {
             outcome = path ? V_P_MISS : V_W_MISS;
         } else {
             HitRec rec;
             PrimView P = load_prim(D.prims, slot);
             float t;
-            hit_prim<true>(P, r, 1e-4f, S.tacc[s], t, &rec);
+            hit_prim<true>(P, r, 1e-4f, ro.w, t, &rec);
             int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
             load_mat(D.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
-            Rng rng(S.rng[s]);
-            stf(S.alb, s, albedo);
+            Rng rng((uint32_t)meta.w);
+            S.alb[s] = f4(albedo);
             float4* SH = S.sh + (size_t)s * kMaxSh;
             uint8_t* OC = S.occl + (size_t)s * kMaxSh;
-            int ns = 0;
-            const f3 T = path ? ldf(S.T, s) : f3(1, 1, 1);
+            const f3 T = path ? xyz(S.T[s]) : f3(1, 1, 1);
             const f3 sn = rec.normal;
             const f3 sh_o = rec.p + rec.normal * st.shadow_bias;
             for (int li = 0; li < D.n_lights; ++li) {
@@ -452,13 +509,10 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
                     ns++;
                 }
             }
-            S.nsh[s] = ns;
             if (path) {
                 // radiance += T * emission comes before the NEE terms (:861)
-                f3 R = ldf(S.R, s);
-                R += T * emission;
-                stf(S.R, s, R);
-                const int depth = S.depth[s];
+                S.R[s] = f4(xyz(S.R[s]) + T * emission);
+                const int depth = ctl.y;
                 outcome = V_P_BOUNCE;
                 f3 Tn = T;
                 if (st.bounces - depth > 3) {   // Russian roulette, :922-926
@@ -480,13 +534,12 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
                         NT = Tn * albedo * refl;
                     } else if (kind == PCL_MAT_DIELECTRIC) {
                         Frame* F = S.frames + (size_t)s * kMaxFrames;
-                        int nf = 0;
+                        nf = 0;
                         Ray rr = push_glass(F, nf, r, rec, albedo, ior, depth, st.shadow_bias);
-                        S.nf[s] = nf;
-                        stf(S.gT, s, Tn);
-                        stf(S.gE, s, emission);
-                        stf(S.ray_o, s, rr.o);
-                        stf(S.ray_d, s, rr.d);
+                        S.gT[s] = f4(Tn);
+                        S.gE[s] = f4(emission);
+                        S.ray_o[s] = f4(rr.o);
+                        S.ray_d[s] = f4(rr.d);
                         outcome = V_P_GLASS;
                     } else {
                         float r1 = rng.unit() * 6.2831853f;
@@ -499,184 +552,180 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
                         nd = (Tt * cosf_cr(r1) * r2s + B * sinf_cr(r1) * r2s + N * sqrtf(1.0f - r2)).normalized();
                     }
                     if (outcome == V_P_BOUNCE) {
-                        stf(S.ray_o, s, rec.p + rec.normal * st.shadow_bias);
-                        stf(S.ray_d, s, nd);
-                        stf(S.NT, s, NT);
+                        S.ray_o[s] = f4(rec.p + rec.normal * st.shadow_bias);
+                        S.ray_d[s] = f4(nd);
+                        S.NT[s] = f4(NT);
                     }
                 }
             } else {
-                stf(S.pend, s, emission);
+                S.pend[s] = f4(emission);
                 outcome = V_W_LEAF;
                 if (kind == PCL_MAT_METAL && refl > 0.0f) {
                     Frame* F = S.frames + (size_t)s * kMaxFrames;
-                    const int nf = S.nf[s];
                     Frame& f = F[nf];
-                    f.type = FR_METAL; f.albedo = albedo; f.w0 = refl; f.depth = S.depth[s];
-                    S.nf[s] = nf + 1;
-                    stf(S.ray_o, s, rec.p + rec.normal * st.shadow_bias);
-                    stf(S.ray_d, s, r.d.normalized().reflect(rec.normal).normalized());
+                    f.type = FR_METAL; f.albedo = albedo; f.w0 = refl; f.depth = ctl.y;
+                    nf += 1;
+                    S.ray_o[s] = f4(rec.p + rec.normal * st.shadow_bias);
+                    S.ray_d[s] = f4(r.d.normalized().reflect(rec.normal).normalized());
                     outcome = V_W_CHILD;
                 } else if (kind == PCL_MAT_DIELECTRIC) {
                     Frame* F = S.frames + (size_t)s * kMaxFrames;
-                    int nf = S.nf[s];
-                    Ray rr = push_glass(F, nf, r, rec, albedo, ior, S.depth[s], st.shadow_bias);
-                    S.nf[s] = nf;
-                    stf(S.ray_o, s, rr.o);
-                    stf(S.ray_d, s, rr.d);
+                    Ray rr = push_glass(F, nf, r, rec, albedo, ior, ctl.y, st.shadow_bias);
+                    S.ray_o[s] = f4(rr.o);
+                    S.ray_d[s] = f4(rr.d);
                     outcome = V_W_CHILD;
-                    S.outcome_aux[s] = 1;   // glass: the local color is discarded
+                    flags |= 1 << 5;   // glass: the local color is discarded
                 }
             }
-            S.rng[s] = rng.s;
+            meta.w = (int)rng.s;
+            S.meta[s] = meta;
         }
-        S.outcome[s] = outcome;
+        S.ctl[s] = make_int4((ctl.x & 15) | flags | (outcome << 8), ctl.y, nf, ns);
     }
 }
 
 // Folds one vertex's light samples exactly as the reference's loops do.
-__device__ f3 fold_whitted_local(const Slots& S, int s, const f3& emission, const f3& albedo, const f3& amb) {
+__device__ f3 fold_whitted_local(const Slots& S, int s, int ns, const f3& emission, const f3& albedo, const f3& amb) {
     const float4* SH = S.sh + (size_t)s * kMaxSh;
     const uint8_t* OC = S.occl + (size_t)s * kMaxSh;
-    const int ns = S.nsh[s];
     f3 local = emission;
     int q = 0;
     while (q < ns) {
-        const int meta = __float_as_int(SH[q].w);
-        const int li = meta & 0xFF, nsamp = (meta >> 8) & 0xFF;
+        const int nsamp = (__float_as_int(SH[q].w) >> 8) & 0xFF;
         f3 contrib(0, 0, 0);
         for (int k = 0; k < nsamp; ++k, ++q) {
             if (OC[q]) continue;
             contrib += f3(SH[q].x, SH[q].y, SH[q].z);
         }
         local += contrib / (float)nsamp;
-        (void)li;
     }
     local += albedo * amb;
     return local;
 }
 
 __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
-    const int cur = C.q.ctr[Q_CUR];
-    const int n = (int)C.q.ctr[Q_COUNT0 + cur];
-    const int32_t* Q = C.q.ray_q[cur];
+    const int cur = C.q.ctr[C_CUR];
+    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
     uint32_t rays = 0;
-    int seg_t = 0, seg_hi = 0;
-    while (true) {
-        const int base = seg_fetch(&C.q.ctr[Q_HEAD_RESOLVE], n, seg_t, seg_hi);
-        if (base < 0) break;
-        const int k = base + lane_id();
+    const int home = wave_gid() & (kStripes - 1);
+    Fetcher f(home), pf(home);
+    int g;
+    uint32_t base, got;
+    while (f.next(C.q.ctr, S_HEAD_RESOLVE, kWave, len, g, base, got)) {
         int a = ADV_IDLE;
         int s = -1;
-        if (k < seg_hi) {
-            s = Q[k];
-            const ItemRec& it = C.items[S.item[s]];
+        Lane L;
+        if ((uint32_t)lane_id() < got) {
+            s = C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()];
+            load_lane(S, s, L);
+            const ItemRec& it = C.items[L.item];
             const DevScene& D = C.scenes[it.scene];
             const f3 amb = ld3(D.amb);
-            const int outcome = S.outcome[s];
             int status;
             f3 result(0, 0, 0);
-            switch (outcome) {
+            switch (L.outcome) {
             case V_W_MISS: status = 3; result = ld3(D.bg); break;
-            case V_W_LEAF: status = 3; result = fold_whitted_local(S, s, ldf(S.pend, s), ldf(S.alb, s), amb); break;
+            case V_W_LEAF: status = 3; result = fold_whitted_local(S, s, L.nsh, xyz(S.pend[s]), xyz(S.alb[s]), amb); break;
             case V_W_CHILD: {
-                if (S.outcome_aux[s]) {
-                    S.outcome_aux[s] = 0;          // glass: local discarded (its draws already made)
+                if (L.outcome_aux) {
+                    L.outcome_aux = 0;            // glass: local discarded (its draws already made)
                 } else {
                     Frame* F = S.frames + (size_t)s * kMaxFrames;
-                    F[S.nf[s] - 1].acc = fold_whitted_local(S, s, ldf(S.pend, s), ldf(S.alb, s), amb);
+                    F[L.nf - 1].acc = fold_whitted_local(S, s, L.nsh, xyz(S.pend[s]), xyz(S.alb[s]), amb);
                 }
-                S.depth[s] = S.depth[s] - 1;
+                L.depth -= 1;
                 status = ADV_QUERY;
                 break;
             }
             case V_P_MISS: {
-                f3 R = ldf(S.R, s);
-                R += ldf(S.T, s) * ld3(D.bg);
-                stf(S.R, s, R);
-                status = finish_sample(C, s, R, rays);
+                L.R += L.T * ld3(D.bg);
+                status = finish_sample(C, L, L.R, rays);
                 break;
             }
             default: {   // path vertex with a hit: NEE terms in light order, then ambient
-                f3 R = ldf(S.R, s);
                 const float4* SH = S.sh + (size_t)s * kMaxSh;
                 const uint8_t* OC = S.occl + (size_t)s * kMaxSh;
-                const int ns = S.nsh[s];
-                for (int q = 0; q < ns; ++q)
-                    if (!OC[q]) R += f3(SH[q].x, SH[q].y, SH[q].z);
-                R += ldf(S.T, s) * ldf(S.alb, s) * amb;
-                stf(S.R, s, R);
-                if (outcome == V_P_END) {
-                    status = finish_sample(C, s, R, rays);
-                } else if (outcome == V_P_GLASS) {
-                    S.mode[s] = M_WHITTED;
-                    S.glass_root[s] = 1;
-                    S.depth[s] = S.depth[s] - 1;   // trace(refl, depth - 1)
+                for (int q = 0; q < L.nsh; ++q)
+                    if (!OC[q]) L.R += f3(SH[q].x, SH[q].y, SH[q].z);
+                L.R += L.T * xyz(S.alb[s]) * amb;
+                if (L.outcome == V_P_END) {
+                    status = finish_sample(C, L, L.R, rays);
+                } else if (L.outcome == V_P_GLASS) {
+                    L.mode = M_WHITTED;
+                    L.glass_root = 1;
+                    L.depth -= 1;                 // trace(refl, depth - 1)
                     status = ADV_QUERY;
                 } else {
-                    stf(S.T, s, ldf(S.NT, s));
-                    S.depth[s] = S.depth[s] - 1;
+                    L.T = xyz(S.NT[s]);
+                    L.depth -= 1;
                     status = ADV_QUERY;
                 }
                 break;
             }
             }
-            a = advance(C, s, status, result, rays);
+            L.outcome = V_NONE;
+            a = advance(C, s, L, status, result, rays);
         }
-        a = refill(C, s, a, rays);
-        const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[Q_COUNT0 + (cur ^ 1)]);
-        if (pos >= 0) C.q.ray_q[cur ^ 1][pos] = s;
+        a = refill(C, s, L, a, pf, rays);
+        if (s >= 0) store_lane(S, s, L);
+        const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + (cur ^ 1), g)]);
+        if (pos >= 0) C.q.ray_q[cur ^ 1][(size_t)g * C.q.qcap + pos] = s;
     }
-    wave_add_u64(rays, &C.q.stats[ST_RAYS]);
+    wave_add_u64(rays, &C.q.stats[(1 + home) * kStatLine + ST_RAYS]);
 }
 
-// Starts the job: every slot takes a pixel and its first query.
+// Starts the job: every slot takes a pixel and its first query.  Slot wave
+// b (slots 64b .. 64b + 63) belongs to stripe b % kStripes.
 __global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
     uint32_t rays = 0;
-    const int cur = C.q.ctr[Q_CUR];
-    for (int base = blockIdx.x * kWfBlock + (threadIdx.x & ~(kWave - 1)); base < n_slots;
-         base += gridDim.x * kWfBlock) {
-        const int s = base + lane_id();
-        int a = ADV_IDLE;
-        if (s < n_slots) {
-            C.sl.nf[s] = 0;
-            C.sl.glass_root[s] = 0;
-            C.sl.outcome_aux[s] = 0;
-            if ((int64_t)s < C.q.total_pixels) {   // slot s starts on job pixel s
-                if (assign_pixel(C, s, (int64_t)s)) {
-                    start_sample(C, s, rays);
-                    a = advance(C, s, ADV_QUERY, f3(0, 0, 0), rays);
-                } else {
-                    a = ADV_NEED_PIXEL;
-                }
-            } else {
-                C.sl.item[s] = -1;
-            }
-        }
-        a = refill(C, s, a, rays);
-        const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[Q_COUNT0 + cur]);
-        if (pos >= 0) C.q.ray_q[cur][pos] = s;
-    }
-    wave_add_u64(rays, &C.q.stats[ST_RAYS]);
+    const int cur = C.q.ctr[C_CUR];
+    const int s0 = wave_gid() * kWave;
+    const int s = s0 + lane_id();
+    const int g = stripe_of(s0);
+    Fetcher pf(g);
+    Lane L;
+    L.item = -1; L.pixel = 0; L.sample = 0; L.rng = 0;
+    L.mode = 0; L.glass_root = 0; L.outcome_aux = 0; L.outcome = V_NONE; L.depth = 0; L.nf = 0; L.nsh = 0;
+    L.o = L.d = L.acc = L.T = L.R = f3(0, 0, 0);
+    int a = (s < n_slots) ? ADV_NEED_PIXEL : ADV_IDLE;
+    a = refill(C, s, L, a, pf, rays);
+    if (s < n_slots) store_lane(C.sl, s, L);
+    const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + cur, g)]);
+    if (pos >= 0) C.q.ray_q[cur][(size_t)g * C.q.qcap + pos] = s;
+    wave_add_u64(rays, &C.q.stats[(1 + g) * kStatLine + ST_RAYS]);
 }
 
-// End of an iteration: account the queries, flip the ray queue, clear heads.
+// End of an iteration, one lane per stripe: account the queries, flip the
+// ray queue, clear the stripe's heads, publish the next queue's length.
 __global__ void k_swap(Queues q) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t cur = q.ctr[Q_CUR];
-    q.stats[ST_CLOSEST] += q.ctr[Q_COUNT0 + cur];
-    q.stats[ST_ANY] += q.ctr[Q_SHADOW];
-    q.stats[ST_ITERS] += q.ctr[Q_COUNT0 + cur] ? 1 : 0;
-    q.ctr[Q_COUNT0 + cur] = 0;
-    q.ctr[Q_CUR] = cur ^ 1;
-    q.ctr[Q_SHADOW] = 0;
-    for (int i = Q_HEAD_CLOSEST; i < Q_NCTR; ++i) q.ctr[i] = 0;
+    const int g = threadIdx.x;
+    const uint32_t cur = q.ctr[C_CUR];
+    uint32_t n = q.ctr[sidx(S_COUNT0 + cur, g)];
+    uint32_t nsh = q.ctr[sidx(S_SHADOW, g)];
+    uint32_t nn = q.ctr[sidx(S_COUNT0 + (cur ^ 1), g)];
+    q.ctr[sidx(S_COUNT0 + cur, g)] = 0;
+    q.ctr[sidx(S_SHADOW, g)] = 0;
+    for (int h = S_HEAD_CLOSEST; h <= S_HEAD_RESOLVE; ++h) q.ctr[sidx(h, g)] = 0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        n += __shfl_xor(n, m, kWave);
+        nsh += __shfl_xor(nsh, m, kWave);
+        nn += __shfl_xor(nn, m, kWave);
+    }
+    if (g == 0) {
+        q.stats[ST_CLOSEST] += n;
+        q.stats[ST_ANY] += nsh;
+        q.stats[ST_ITERS] += n ? 1 : 0;
+        q.ctr[C_CUR] = cur ^ 1;
+        q.ctr[C_PENDING] = nn;
+    }
 }
 
 // Clears the job counters (a kernel, not a memset, so it may sit in a graph).
-__global__ void k_reset(Queues q, uint32_t first_free_pixel) {
-    if (threadIdx.x < Q_NCTR) q.ctr[threadIdx.x] = threadIdx.x == Q_PIXEL ? first_free_pixel : 0u;
-    if (threadIdx.x < ST_N) q.stats[threadIdx.x] = 0;
+__global__ void k_reset(Queues q) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kCtrWords; i += gridDim.x * blockDim.x) q.ctr[i] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kStatWords; i += gridDim.x * blockDim.x) q.stats[i] = 0;
 }
 
 }  // namespace wf
@@ -708,7 +757,6 @@ void Engine::release() {
     bufs.clear();
     cap_slots = 0;
     cap_items = 0;
-    cap_shadow = 0;
 }
 
 int Engine::alloc(size_t bytes, void** p, std::string& err) {
@@ -726,22 +774,22 @@ int Engine::ensure(int slots, int items, std::string& err) {
 #define A(field, type, count)                                        \
     if ((rc = alloc(sizeof(type) * (count), &p, err)) != PCL_OK) return rc; \
     field = static_cast<type*>(p);
-    A(sl.item, int32_t, S) A(sl.pixel, int32_t, S) A(sl.rng, uint32_t, S) A(sl.sample, int32_t, S)
-    A(sl.acc, float, 3 * S) A(sl.mode, int32_t, S) A(sl.depth, int32_t, S) A(sl.nf, int32_t, S)
-    A(sl.ray_o, float, 3 * S) A(sl.ray_d, float, 3 * S) A(sl.hit, int32_t, S) A(sl.tacc, float, S)
-    A(sl.T, float, 3 * S) A(sl.NT, float, 3 * S) A(sl.R, float, 3 * S) A(sl.pend, float, 3 * S)
-    A(sl.alb, float, 3 * S) A(sl.gT, float, 3 * S) A(sl.gE, float, 3 * S) A(sl.outcome, int32_t, S)
-    A(sl.outcome_aux, int32_t, S) A(sl.glass_root, int32_t, S) A(sl.nsh, int32_t, S)
+    A(sl.meta, int4, S) A(sl.ctl, int4, S) A(sl.ray_o, float4, S) A(sl.ray_d, float4, S)
+    A(sl.acc, float4, S) A(sl.T, float4, S) A(sl.NT, float4, S) A(sl.R, float4, S) A(sl.pend, float4, S)
+    A(sl.alb, float4, S) A(sl.gT, float4, S) A(sl.gE, float4, S)
     A(sl.sh, float4, S * kMaxSh) A(sl.occl, uint8_t, S * kMaxSh)
     A(sl.frames, Frame, S * kMaxFrames)
-    A(q.ray_q[0], int32_t, S) A(q.ray_q[1], int32_t, S)
-    A(q.sh_who, int2, S * kMaxSh) A(q.sh_ray, float4, 2 * S * kMaxSh)
-    A(q.ctr, uint32_t, 64) A(q.stats, unsigned long long, 16)
+    const size_t qcap = ((S + 63) / 64 + kStripes - 1) / kStripes * 64;   // a stripe's slots
+    const size_t Q = qcap * kStripes, SH = Q * kMaxSh;
+    A(q.ray_q[0], int32_t, Q) A(q.ray_q[1], int32_t, Q)
+    A(q.sh_who, int2, SH) A(q.sh_ray, float4, 2 * SH)
+    A(q.ctr, uint32_t, kCtrWords) A(q.stats, unsigned long long, kStatWords)
+    q.qcap = (int32_t)qcap;
+    q.shcap = (int32_t)(qcap * kMaxSh);
     A(d_items, ItemRec, items) A(d_scenes, DevScene, items)
 #undef A
     cap_slots = slots;
     cap_items = items;
-    cap_shadow = slots * kMaxSh;
     return PCL_OK;
 }
 
@@ -751,7 +799,7 @@ static void launch_iter(const Ctx& c, int grid_trav, int grid_other, hipStream_t
     hipLaunchKernelGGL(k_shade, dim3(grid_other), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_anyhit<K>, dim3(grid_trav), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_resolve, dim3(grid_other), dim3(kWfBlock), 0, st, c);
-    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, st, c.q);
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(kStripes), 0, st, c.q);
 }
 
 int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
@@ -805,26 +853,27 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     WF_TRY(hipEventCreate(&e0));
     WF_TRY(hipEventCreate(&e1));
     WF_TRY(hipEventRecord(e0, stream));
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, c.q, (uint32_t)std::min<int64_t>(total, cfg.slots));
+    hipLaunchKernelGGL(k_reset, dim3(16), dim3(256), 0, stream, c.q);
     hipLaunchKernelGGL(k_start, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c, cfg.slots);
     WF_TRY(hipGetLastError());
-    uint32_t h_ctr[Q_NCTR];
+    uint32_t h_pending = 0;
     int launches = 0;
     const double t_begin = now_ms();
     while (true) {
         WF_TRY(hipGraphLaunch(graph_exec, stream));
         ++launches;
-        WF_TRY(hipMemcpyAsync(h_ctr, q.ctr, sizeof(h_ctr), hipMemcpyDeviceToHost, stream));
+        WF_TRY(hipMemcpyAsync(&h_pending, q.ctr + C_PENDING, sizeof(h_pending), hipMemcpyDeviceToHost, stream));
         WF_TRY(hipStreamSynchronize(stream));
-        if (h_ctr[Q_COUNT0 + h_ctr[Q_CUR]] == 0) break;
+        if (h_pending == 0) break;
         // every iteration retires at least one query of a finite program; a
         // job still running after this long is a bug, reported, not waited on
         if (now_ms() - t_begin > 300000.0) { err = "wavefront did not drain in 300 s"; return PCL_ERR_LIMIT; }
     }
     WF_TRY(hipEventRecord(e1, stream));
-    unsigned long long h_st[ST_N];
-    WF_TRY(hipMemcpyAsync(h_st, q.stats, sizeof(h_st), hipMemcpyDeviceToHost, stream));
+    std::vector<unsigned long long> h_st(kStatWords);
+    WF_TRY(hipMemcpyAsync(h_st.data(), q.stats, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     WF_TRY(hipStreamSynchronize(stream));
+    for (int g = 0; g < kStripes; ++g) h_st[ST_RAYS] += h_st[(1 + g) * kStatLine + ST_RAYS];
     float ms = 0.0f;
     WF_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);

@@ -26,31 +26,48 @@ struct ItemRec {
     int64_t pix_begin;   // first job pixel index of this item
 };
 
-// Pixel-slot state, structure of arrays (one entry per slot).
+// Pixel-slot state, structure of arrays of 16-byte records (one per slot),
+// so a kernel loads and stores a slot's state with a few dwordx4 accesses.
 struct Slots {
-    int32_t *item, *pixel, *sample, *mode, *depth, *nf, *hit, *outcome, *outcome_aux, *glass_root, *nsh;
-    uint32_t* rng;
-    float *acc, *ray_o, *ray_d, *tacc, *T, *NT, *R, *pend, *alb, *gT, *gE;
+    int4* meta;          // item, pixel, sample index, rng state
+    int4* ctl;           // mode | glass_root<<4 | glass-discard<<5 | outcome<<8, depth, frames, samples
+    float4 *ray_o, *ray_d;   // current ray; after a closest query .w = t_max at acceptance / hit slot
+    float4 *acc, *T, *NT, *R, *pend, *alb, *gT, *gE;
     float4* sh;          // [slot][kMaxSh] light-sample contribution + meta
-    uint8_t* occl;      // [slot][kMaxSh] occluded flags
+    uint8_t* occl;       // [slot][kMaxSh] occluded flags
     Frame* frames;       // [slot][kMaxFrames]
 };
 
-constexpr int kXcds = 8;   // MI355X: 8 XCDs, each with its own L2
+// Work queues and counters are split into kStripes stripes.  Slot s belongs
+// to stripe (s / 64) % kStripes for its whole life, and every queue holds a
+// stripe's slots in that stripe's own segment, so each stripe's counters get
+// 1/kStripes of the atomics.  Every counter sits in its own 128-byte line:
+// device-scope atomics on one line serialize (about 11 ns each on MI355X),
+// atomics on different lines proceed in parallel.
+constexpr int kStripes = 64;   // equal to the wave size: a scan reads one stripe per lane
+constexpr int kLine = 32;      // uint32 words per 128-byte line
+enum : int { C_CUR = 0, C_PENDING = 1 };   // global counters, line 0
 enum : int {
-    Q_CUR = 0, Q_COUNT0 = 1, Q_COUNT1 = 2, Q_SHADOW = 3, Q_PIXEL = 4,
-    // per-stage work heads, one per XCD segment of the queue
-    Q_HEAD_CLOSEST = 16, Q_HEAD_SHADE = 24, Q_HEAD_ANY = 32, Q_HEAD_RESOLVE = 40, Q_NCTR = 48
+    S_COUNT0 = 0, S_COUNT1 = 1, S_SHADOW = 2,   // queue lengths per stripe
+    S_HEAD_CLOSEST = 3, S_HEAD_SHADE = 4, S_HEAD_ANY = 5, S_HEAD_RESOLVE = 6,   // work heads
+    S_PIXEL = 7,   // next job pixel of the stripe's share of the job
+    S_N = 8
 };
-enum : int { ST_RAYS = 0, ST_CLOSEST = 1, ST_ANY = 2, ST_ITERS = 3, ST_N = 8 };
+PCL_HD int sidx(int type, int g) { return (1 + type * kStripes + g) * kLine; }
+constexpr int kCtrWords = (1 + S_N * kStripes) * kLine;
+// statistics: line 0 global (written by the swap kernel), then one line per stripe
+enum : int { ST_RAYS = 0, ST_CLOSEST = 1, ST_ANY = 2, ST_ITERS = 3 };
+constexpr int kStatLine = 16;   // u64 per line
+constexpr int kStatWords = (1 + kStripes) * kStatLine;
 
 struct Queues {
-    int32_t* ray_q[2];        // slot ids awaiting a closest-hit query
-    int2* sh_who;             // shadow queue: (slot, sample index)
-    float4* sh_ray;           // shadow queue: (o, t_min), (d, t_max)
-    uint32_t* ctr;            // Q_* counters
+    int32_t* ray_q[2];        // slot ids awaiting a closest-hit query, [stripe][qcap]
+    int2* sh_who;             // shadow queue: (slot, sample index), [stripe][shcap]
+    float4* sh_ray;           // shadow queue: (o, t_min), (d, t_max) pairs
+    uint32_t* ctr;            // counters, see sidx()
     unsigned long long* stats;
-    int64_t total_pixels;
+    int32_t qcap, shcap;      // per-stripe capacities
+    int64_t total_pixels;     // job pixels (a multiple of 64)
 };
 
 struct Config {
@@ -72,7 +89,7 @@ struct Engine {
     Queues q{};
     ItemRec* d_items = nullptr;
     DevScene* d_scenes = nullptr;
-    int cap_slots = 0, cap_items = 0, cap_shadow = 0;
+    int cap_slots = 0, cap_items = 0;
     std::vector<void*> bufs;
     hipGraphExec_t graph_exec = nullptr;
     Config graph_key;
