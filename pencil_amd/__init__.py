@@ -17,7 +17,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-HIP_LIB = os.path.join(_HERE, "libpencil_hip.so")
+# PENCIL_AMD_HIP_LIB selects another build of the core (other compile options).
+HIP_LIB = os.environ.get("PENCIL_AMD_HIP_LIB") or os.path.join(_HERE, "libpencil_hip.so")
 HOST_LIB = os.path.join(_HERE, "libpencil_host.so")
 CLI = os.path.join(_HERE, "bin", "pencil")
 

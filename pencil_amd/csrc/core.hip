@@ -82,8 +82,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
     if (COUNT) flush_counts(c, ctr);
 }
 
+// At most 168 VGPRs, 3 waves per SIMD: measured faster than both the
+// unconstrained 183 (2 waves) and 128 (4 waves, more spills).
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
+__global__ __launch_bounds__(kBlock, 3) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
                                                    unsigned long long* ctr) {
     __shared__ unsigned long long lds[kStackK * kBlock];
     // Tiles in row-major block order.  (Giving each of the 8 XCDs a contiguous
@@ -584,7 +586,7 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
         if (v < total) slot_opts.push_back((int)v);
     slot_opts.push_back((int)std::min<int64_t>(total, 1 << 23));
     for (int knob = 0; knob < 4; ++knob) {
-        std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 8, 16}
+        std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 6, 10, 12, 16}
                               : knob == 2 ? std::vector<int>{8, 16, 32} : std::vector<int>{1, 4, 16};
         for (int v : opts) {
             pcl::wf::Config c = cur;

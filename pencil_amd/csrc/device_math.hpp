@@ -68,10 +68,14 @@ struct f3 {
 PCL_HD float clampf(float v, float lo, float hi) { return (v < lo) ? lo : (v > hi) ? hi : v; }
 
 // libm-equivalent transcendentals: one double evaluation, one rounding.
-PCL_D float cosf_cr(float x) { return (float)cos((double)x); }
-PCL_D float sinf_cr(float x) { return (float)sin((double)x); }
-PCL_D float powf_cr(float x, float y) { return (float)pow((double)x, (double)y); }
-PCL_D float atan2f_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
+// Kept out of line: each double-precision expansion is a few KB of code, and
+// inlined at every call site it would push the kernels far past the
+// instruction cache.
+#define PCL_DN __device__ __attribute__((noinline))
+PCL_DN float cosf_cr(float x) { return (float)cos((double)x); }
+PCL_DN float sinf_cr(float x) { return (float)sin((double)x); }
+PCL_DN float powf_cr(float x, float y) { return (float)pow((double)x, (double)y); }
+PCL_DN float atan2f_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
 
 // schlick (src/vec3.hpp:83-87)
 PCL_D float schlick(float c, float r) {

@@ -129,7 +129,7 @@ struct Fetcher {
             const int mine = lane_id();
             const bool has = ld_live(&ctr[sidx(head, mine)]) < len(mine);
             const unsigned long long m = __ballot(has) & ~dead;
-            if (!m) return false;
+            if (!m) { dead = ~0ull; return false; }   // heads only grow: for good
             const unsigned long long r = rot ? ((m >> rot) | (m << (kWave - rot))) : m;
             const int pick = (rot + __ffsll((long long)r) - 1) & (kStripes - 1);
             const uint32_t L = len(pick);
@@ -203,7 +203,8 @@ __device__ __forceinline__ void store_lane(const Slots& S, int s, const Lane& L)
 }
 
 // Camera ray of the lane's current sample (render_tile_region :361-367).
-__device__ void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
+// Out of line: it is reached from several places of every slot kernel.
+PCL_DN void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
     const ItemRec& it = C.items[L.item];
     const RSettings& st = it.st;
     const int W = st.width, H = st.height, n = st.samples;
@@ -828,7 +829,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     c.q = q;
     c.q.total_pixels = total;
     const int grid_trav = cus * cfg.blocks_per_cu;
-    const int grid_other = cus * 4;
+    const int grid_other = cus * 8;   // 4 waves per SIMD, the register limit of shade / resolve
     const bool graph_ok = (graph_exec && graph_key.slots == cfg.slots && graph_key.blocks_per_cu == cfg.blocks_per_cu &&
                            graph_key.stack_k == cfg.stack_k && graph_key.iters_per_graph == cfg.iters_per_graph &&
                            graph_items == (int)its.size() && graph_total == total);
