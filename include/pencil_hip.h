@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 1
+#define PCL_ABI_VERSION 2
 
 enum {
     PCL_OK              = 0,
@@ -262,6 +262,40 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
  * knobs) and returns the fastest; also remembered as the context default. */
 int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best,
                     double* best_ms);
+
+/* ---- differentiation: adjoint (path-replay backprop) ----------------------
+ * Parameters are every material's albedo (RGB) and emission (RGB), laid out
+ * grad[m*6 + 0..2] = d/d albedo_m, grad[m*6 + 3..5] = d/d emission_m, the
+ * order in which a caller perturbs Material::albedo / Material::emission
+ * (src/material.hpp:29-60) for finite differences.  The adjoint pass replays
+ * every pixel's samples on the pixel's own RNG stream (the same paths as the
+ * forward render) and pushes the image adjoint down them; per-pixel partials
+ * are accumulated in FP64 and reduced in a fixed order, so the gradient is
+ * bit-reproducible run to run.  Russian roulette's 1/p weight is held
+ * constant (its branch is discrete).  At most PCL_ADJ_MAX_MATERIALS
+ * materials.  No reference counterpart: the reference has no derivatives; its
+ * gradient is estimated by central differences of whole renders. */
+#define PCL_ADJ_MAX_MATERIALS 64
+
+/* Replaces the scene's material table (same number of materials, same kinds).
+ * BVH, primitives and lights are kept.  This is the only state an inverse-
+ * rendering step changes. */
+int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* materials, int32_t n);
+
+/* Vector-Jacobian product of the rendered image: grad[p] = sum over pixels and
+ * channels of d_adj[px*3+c] * dI[px*3+c]/dp.  d_adj: device, W*H*3 floats.
+ * grad: host, n_materials*6 doubles. */
+int pcl_render_vjp_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                          const float* d_adj, double* grad, pcl_stats* stats);
+
+/* L2 image loss  loss = sum_px,c (I - target)^2  of a forward render and its
+ * gradient by the adjoint pass.  target_rgb: host W*H*3 floats; grad: host
+ * n_materials*6 doubles; loss: host, one double; out_rgb: optional host image. */
+int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                const float* target_rgb, double* grad, double* loss, float* out_rgb, pcl_stats* stats);
+/* Same with the target (and the optional image output) in device memory. */
+int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                       const float* d_target, double* grad, double* loss, float* d_out_rgb, pcl_stats* stats);
 
 /* Device-memory helpers for callers without their own allocator. */
 int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr);

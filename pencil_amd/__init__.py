@@ -139,6 +139,13 @@ def hip_lib():
              [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(BatchStats)])
         _sig(lib, "pcl_wf_autotune", C.c_int,
              [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(C.c_double)])
+        _sig(lib, "pcl_scene_set_materials", C.c_int, [_P, C.c_int32, C.POINTER(Material), C.c_int32])
+        _sig(lib, "pcl_render_vjp_device", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_adjoint", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, _P, _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_adjoint_device", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, _P, _P, C.POINTER(Stats)])
         _sig(lib, "pcl_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)])
         _sig(lib, "pcl_device_free", C.c_int, [_P, _P])
         _sig(lib, "pcl_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t])
@@ -242,6 +249,27 @@ class HostScene:
             pass
 
 
+def materials_of(desc: SceneDesc):
+    """A copy of a scene description's material table."""
+    arr = (Material * max(1, desc.n_materials))()
+    for i in range(desc.n_materials):
+        C.memmove(C.byref(arr[i]), C.byref(desc.materials[i]), C.sizeof(Material))
+    return arr, desc.n_materials
+
+
+def material_params(mats, n) -> np.ndarray:
+    """(n, 6) float64: albedo rgb, emission rgb per material."""
+    return np.array([list(mats[i].albedo) + list(mats[i].emission) for i in range(n)], dtype=np.float64)
+
+
+def set_material_params(mats, n, params: np.ndarray):
+    """Writes (n, 6) albedo / emission values into a material table (as float32)."""
+    for i in range(n):
+        for c in range(3):
+            mats[i].albedo[c] = float(np.float32(params[i, c]))
+            mats[i].emission[c] = float(np.float32(params[i, 3 + c]))
+
+
 def tonemap_rgb8(hdr: np.ndarray, exposure: float = 0.0) -> np.ndarray:
     """Host ACES + gamma 2.2 + 8-bit encode (the CLI's write_image path)."""
     hdr = np.ascontiguousarray(hdr, dtype=np.float32)
@@ -281,7 +309,44 @@ class Core:
     def upload(self, desc: SceneDesc) -> int:
         sid = C.c_int32(-1)
         self._check(self.lib.pcl_scene_upload(self.ctx, C.byref(desc), C.byref(sid)), "pcl_scene_upload")
+        self._nmat = getattr(self, "_nmat", {})
+        self._nmat[sid.value] = desc.n_materials
         return sid.value
+
+    def set_materials(self, sid: int, mats, n: int):
+        self._check(self.lib.pcl_scene_set_materials(self.ctx, sid, mats, n), "pcl_scene_set_materials")
+
+    def adjoint(self, sid: int, cam: Camera, settings: Settings, target: np.ndarray):
+        """L2 loss against target and its gradient: (grad (n_mat, 6), loss, image, Stats)."""
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        n = self._nmat[sid]
+        grad = np.zeros((max(n, 1), 6), dtype=np.float64)
+        loss = C.c_double()
+        img = np.empty((settings.height, settings.width, 3), dtype=np.float32)
+        st = Stats()
+        self._check(self.lib.pcl_adjoint(self.ctx, sid, C.byref(cam), C.byref(settings), _ptr(target), _ptr(grad),
+                                         C.byref(loss), _ptr(img), C.byref(st)), "pcl_adjoint")
+        return grad[:n], loss.value, img, st
+
+    def adjoint_device(self, sid: int, cam: Camera, settings: Settings, d_target: int, d_out: int = 0):
+        n = self._nmat[sid]
+        grad = np.zeros((max(n, 1), 6), dtype=np.float64)
+        loss = C.c_double()
+        st = Stats()
+        self._check(self.lib.pcl_adjoint_device(self.ctx, sid, C.byref(cam), C.byref(settings), C.c_void_p(d_target),
+                                                _ptr(grad), C.byref(loss), C.c_void_p(d_out or None), C.byref(st)),
+                    "pcl_adjoint_device")
+        return grad[:n], loss.value, st
+
+    def render_vjp_device(self, sid: int, cam: Camera, settings: Settings, d_adj: int):
+        """grad (n_mat, 6) of sum(adj * image) for a device adjoint image."""
+        n = self._nmat[sid]
+        grad = np.zeros((max(n, 1), 6), dtype=np.float64)
+        st = Stats()
+        self._check(self.lib.pcl_render_vjp_device(self.ctx, sid, C.byref(cam), C.byref(settings),
+                                                   C.c_void_p(d_adj), _ptr(grad), C.byref(st)),
+                    "pcl_render_vjp_device")
+        return grad[:n], st
 
     def free_scene(self, sid: int):
         self._check(self.lib.pcl_scene_free(self.ctx, sid), "pcl_scene_free")

@@ -1,0 +1,129 @@
+// adjoint.hip - the adjoint pass: per-pixel replay with gradient
+// accumulation, and the deterministic FP64 reduction of the partials.
+//
+// k_adjoint runs one 8x8 pixel tile per 64-lane block.  Each lane replays its
+// pixel (render_pixel's program with the adjoint of adjoint.hpp) into a
+// private FP64 accumulator, then the block sums its 64 lanes with a fixed
+// butterfly and writes one partial per parameter.  k_reduce_params sums the
+// block partials of each parameter in block order with a fixed tree.  No
+// atomics touch a gradient, so the result is the same on every run.
+#include <hip/hip_runtime.h>
+
+#include "adjoint.hpp"
+#include "adjoint_launch.hpp"
+
+namespace pcl {
+namespace {
+
+constexpr int kAdjBlock = 64;
+constexpr int kReduceBlock = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// adj != nullptr: the image adjoint is given (VJP).  Otherwise it is the L2
+// loss's, 2 (img - tgt), and the loss itself is reduced into the last slot.
+template <int NM>
+__global__ __launch_bounds__(kAdjBlock) void k_adjoint(DevScene S, CamRec C, RSettings st, const float* img,
+                                                       const float* tgt, const float* adj, double* partial,
+                                                       int n_par) {
+    __shared__ unsigned long long lds[kStackK * kAdjBlock];
+    const int tiles_x = (st.width + 7) / 8;
+    const int lane = threadIdx.x;
+    const int i = (blockIdx.x % tiles_x) * 8 + (lane & 7);
+    const int j = (blockIdx.x / tiles_x) * 8 + (lane >> 3);
+    Grad<NM> G;
+    G.clear();
+    double loss = 0.0;
+    if (i < st.width && j < st.height) {
+        const size_t px = ((size_t)j * st.width + i) * 3;
+        f3 Wp;
+        if (adj) {
+            Wp = f3(adj[px], adj[px + 1], adj[px + 2]);
+        } else {
+            const f3 d(img[px] - tgt[px], img[px + 1] - tgt[px + 1], img[px + 2] - tgt[px + 2]);
+            loss = (double)d.x * d.x + (double)d.y * d.y + (double)d.z * d.z;
+            Wp = d * 2.0f;
+        }
+        const int n = st.samples;
+        const f3 W = Wp / (float)(n * n);   // the image is the mean of n*n samples
+        AdjFrame F[kMaxFrames];
+        Stack stk;
+        stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
+        stk.stride = kAdjBlock;
+        LaneCount<false> c;
+        // render_pixel (render.hpp), replayed
+        const int Wd = st.width, Hd = st.height;
+        Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
+        for (int sy = 0; sy < n; ++sy) {
+            for (int sx = 0; sx < n; ++sx) {
+                float u = (i + (sx + 0.5f) / n) / (float)Wd;
+                float v = 1.0f - (j + (sy + 0.5f) / n) / (float)Hd;
+                u += rng.sym() * (0.5f / (Wd * n));
+                v += rng.sym() * (0.5f / (Hd * n));
+                Ray r = camera_ray(C, u, v, rng);
+                if (st.pathtrace) (void)trace_path_adj(S, st, r, rng, F, stk, c, W, G);
+                else (void)whitted_adj(S, st, r, st.max_depth, rng, F, 0, stk, c, W, G);
+            }
+        }
+    }
+    double* out = partial + (size_t)blockIdx.x * (n_par + 1);
+    for (int p = 0; p < n_par; ++p) {
+        const double v = wave_sum(G.g[p]);
+        if (lane == 0) out[p] = v;
+    }
+    const double l = wave_sum(loss);
+    if (lane == 0) out[n_par] = l;
+}
+
+// out[p] = sum over blocks b (in order of b within each thread's stride, then
+// a fixed tree over threads) of partial[b][p].
+__global__ __launch_bounds__(kReduceBlock) void k_reduce_params(const double* partial, int n_blocks, int stride,
+                                                               double* out) {
+    __shared__ double sh[kReduceBlock];
+    const int p = blockIdx.x;
+    double v = 0.0;
+    for (int b = threadIdx.x; b < n_blocks; b += kReduceBlock) v += partial[(size_t)b * stride + p];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kReduceBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[p] = sh[0];
+}
+
+template <int NM>
+void launch_nm(hipStream_t s, const DevScene& D, const CamRec& C, const RSettings& st, int tiles, const float* img,
+               const float* tgt, const float* adj, double* partial, int n_par) {
+    hipLaunchKernelGGL(k_adjoint<NM>, dim3(tiles), dim3(kAdjBlock), 0, s, D, C, st, img, tgt, adj, partial, n_par);
+}
+
+}  // namespace
+
+namespace adj {
+
+int tiles(const RSettings& st) { return ((st.width + 7) / 8) * ((st.height + 7) / 8); }
+
+size_t partial_doubles(const RSettings& st, int n_mat) { return (size_t)tiles(st) * (size_t)(n_mat * 6 + 1); }
+
+hipError_t launch(hipStream_t s, const DevScene& D, const CamRec& C, const RSettings& st, int n_mat,
+                  const float* img, const float* tgt, const float* adjw, double* partial, double* out) {
+    const int t = tiles(st);
+    const int n_par = n_mat * 6;
+    if (n_mat <= 4) launch_nm<4>(s, D, C, st, t, img, tgt, adjw, partial, n_par);
+    else if (n_mat <= 8) launch_nm<8>(s, D, C, st, t, img, tgt, adjw, partial, n_par);
+    else if (n_mat <= 16) launch_nm<16>(s, D, C, st, t, img, tgt, adjw, partial, n_par);
+    else if (n_mat <= 32) launch_nm<32>(s, D, C, st, t, img, tgt, adjw, partial, n_par);
+    else launch_nm<kAdjMaxMats>(s, D, C, st, t, img, tgt, adjw, partial, n_par);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_reduce_params, dim3(n_par + 1), dim3(kReduceBlock), 0, s, partial, t, n_par + 1, out);
+    return hipGetLastError();
+}
+
+}  // namespace adj
+}  // namespace pcl

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint_launch.hpp"
 #include "host_build.hpp"
 #include "pencil_hip.h"
 #include "render.hpp"
@@ -115,6 +116,7 @@ struct SceneSlot {
     std::vector<void*> bufs;
     std::vector<int32_t> leaf_order;
     std::vector<LightRec> lights;   // host copy, for per-item sample budgets
+    int32_t n_mats = 0;
     pcl_scene_info info{};
 };
 
@@ -130,6 +132,13 @@ struct pcl_ctx {
     std::vector<SceneSlot> scenes;
     pcl::wf::Engine* wf = nullptr;
     pcl::wf::Config wf_default;      // set by pcl_wf_autotune
+    // adjoint work buffers, grown on demand
+    double* d_adj_partial = nullptr;
+    size_t adj_partial_cap = 0;
+    double* d_adj_out = nullptr;     // kAdjMaxMats*6 + 1 doubles
+    float* d_img = nullptr;          // forward image / target staging
+    float* d_tgt = nullptr;
+    size_t img_cap = 0;
 };
 
 namespace {
@@ -238,6 +247,10 @@ void pcl_destroy(pcl_ctx* ctx) {
     for (auto& s : ctx->scenes) if (s.used) release(s);
     delete ctx->wf;
     if (ctx->d_ctr) (void)hipFree(ctx->d_ctr);
+    if (ctx->d_adj_partial) (void)hipFree(ctx->d_adj_partial);
+    if (ctx->d_adj_out) (void)hipFree(ctx->d_adj_out);
+    if (ctx->d_img) (void)hipFree(ctx->d_img);
+    if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -283,6 +296,7 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     for (int k = 0; k < 3; ++k) { D.bg[k] = desc->background[k]; D.amb[k] = desc->ambient[k]; }
     sl.leaf_order = std::move(L.leaf_order);
     sl.lights = L.lights;
+    sl.n_mats = desc->n_materials;
     sl.info.n_prims = desc->n_prims;
     sl.info.n_nodes = (int32_t)L.nodes.size();
     sl.info.depth = L.depth;
@@ -599,6 +613,153 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     ctx->wf_default = cur;
     if (best) *best = to_abi(cur);
     if (best_ms) *best_ms = cur_ms;
+    return PCL_OK;
+}
+
+int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* materials, int32_t n) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (!materials || n != sl->n_mats)
+        return fail(ctx, PCL_ERR_INVALID, "set_materials: expected " + std::to_string(sl->n_mats) + " materials");
+    std::vector<MatRec> recs;
+    std::string err;
+    const int rc = pack_materials(materials, n, recs, err);
+    if (rc) return fail(ctx, rc, err);
+    if (n == 0) return PCL_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(const_cast<MatRec*>(sl->dev.mats), recs.data(), recs.size() * sizeof(MatRec),
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PCL_OK;
+}
+
+// Grows the adjoint's partials buffer to this launch and makes sure the
+// output buffer exists.
+static int adj_buffers(pcl_ctx* ctx, const RSettings& st, int n_mat) {
+    const size_t need = pcl::adj::partial_doubles(st, n_mat);
+    if (need > ctx->adj_partial_cap) {
+        if (ctx->d_adj_partial) (void)hipFree(ctx->d_adj_partial);
+        ctx->d_adj_partial = nullptr;
+        ctx->adj_partial_cap = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_adj_partial, need * sizeof(double)));
+        ctx->adj_partial_cap = need;
+    }
+    if (!ctx->d_adj_out) HIP_TRY(ctx, hipMalloc(&ctx->d_adj_out, (PCL_ADJ_MAX_MATERIALS * 6 + 1) * sizeof(double)));
+    return PCL_OK;
+}
+
+static int adj_check(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const pcl_settings* settings,
+                     double* grad) {
+    if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    if (!cam || !grad) return fail(ctx, PCL_ERR_INVALID, "null camera or gradient output");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (sl->n_mats > PCL_ADJ_MAX_MATERIALS)
+        return fail(ctx, PCL_ERR_LIMIT, "adjoint: " + std::to_string(sl->n_mats) + " materials, at most " +
+                                            std::to_string(PCL_ADJ_MAX_MATERIALS));
+    return PCL_OK;
+}
+
+// Adjoint pass + reduction; copies n_mat*6 (+ the loss) doubles back.
+static int adj_run(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const pcl_settings* settings,
+                   const float* img, const float* tgt, const float* adjw, double* grad, double* loss,
+                   float* adj_ms) {
+    const CamRec C = make_camera(*cam);
+    const RSettings st = to_dev(settings);
+    int rc = adj_buffers(ctx, st, sl->n_mats);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, pcl::adj::launch(ctx->stream, sl->dev, C, st, sl->n_mats, img, tgt, adjw, ctx->d_adj_partial,
+                                  ctx->d_adj_out));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<double> h((size_t)sl->n_mats * 6 + 1);
+    HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->d_adj_out, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipEventElapsedTime(adj_ms, ctx->ev0, ctx->ev1));
+    for (int p = 0; p < sl->n_mats * 6; ++p) grad[p] = h[p];
+    if (loss) *loss = h[(size_t)sl->n_mats * 6];
+    return PCL_OK;
+}
+
+int pcl_render_vjp_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                          const float* d_adj, double* grad, pcl_stats* stats) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    int rc = adj_check(ctx, sl, cam, settings, grad);
+    if (rc) return rc;
+    if (!d_adj) return fail(ctx, PCL_ERR_INVALID, "null image adjoint");
+    const double t0 = now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    float ms = 0.0f;
+    rc = adj_run(ctx, sl, cam, settings, nullptr, nullptr, d_adj, grad, nullptr, &ms);
+    if (rc) return rc;
+    if (stats) {
+        *stats = pcl_stats{};
+        stats->kernel_ms = ms;
+        stats->total_ms = now_ms() - t0;
+    }
+    return PCL_OK;
+}
+
+int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                       const float* d_target, double* grad, double* loss, float* d_out_rgb, pcl_stats* stats) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    int rc = adj_check(ctx, sl, cam, settings, grad);
+    if (rc) return rc;
+    if (!d_target) return fail(ctx, PCL_ERR_INVALID, "null target");
+    const double t0 = now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    float* img = d_out_rgb;
+    if (!img) {   // forward image into the context's staging buffer
+        const size_t n = (size_t)settings->width * settings->height * 3;
+        if (n > ctx->img_cap) {
+            if (ctx->d_img) (void)hipFree(ctx->d_img);
+            if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
+            ctx->d_img = ctx->d_tgt = nullptr;
+            ctx->img_cap = 0;
+            HIP_TRY(ctx, hipMalloc(&ctx->d_img, n * sizeof(float)));
+            HIP_TRY(ctx, hipMalloc(&ctx->d_tgt, n * sizeof(float)));
+            ctx->img_cap = n;
+        }
+        img = ctx->d_img;
+    }
+    pcl_stats fst{};
+    rc = render_impl(ctx, scene, cam, settings, img, &fst, nullptr);
+    if (rc) return rc;
+    float ms = 0.0f;
+    rc = adj_run(ctx, sl, cam, settings, img, d_target, nullptr, grad, loss, &ms);
+    if (rc) return rc;
+    if (stats) {
+        *stats = fst;
+        stats->kernel_ms = fst.kernel_ms + ms;
+        stats->total_ms = now_ms() - t0;
+    }
+    return PCL_OK;
+}
+
+int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                const float* target_rgb, double* grad, double* loss, float* out_rgb, pcl_stats* stats) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    int rc = adj_check(ctx, sl, cam, settings, grad);
+    if (rc) return rc;
+    if (!target_rgb) return fail(ctx, PCL_ERR_INVALID, "null target");
+    const double t0 = now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)settings->width * settings->height * 3;
+    if (n > ctx->img_cap) {
+        if (ctx->d_img) (void)hipFree(ctx->d_img);
+        if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
+        ctx->d_img = ctx->d_tgt = nullptr;
+        ctx->img_cap = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_img, n * sizeof(float)));
+        HIP_TRY(ctx, hipMalloc(&ctx->d_tgt, n * sizeof(float)));
+        ctx->img_cap = n;
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tgt, target_rgb, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    rc = pcl_adjoint_device(ctx, scene, cam, settings, ctx->d_tgt, grad, loss, ctx->d_img, stats);
+    if (rc) return rc;
+    if (out_rgb) HIP_TRY(ctx, hipMemcpy(out_rgb, ctx->d_img, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (stats) stats->total_ms = now_ms() - t0;
     return PCL_OK;
 }
 

@@ -183,6 +183,24 @@ struct Builder {
 
 }  // namespace
 
+int pack_materials(const pcl_material* mats, int n, std::vector<MatRec>& out, std::string& err) {
+    out.clear();
+    for (int i = 0; i < n; ++i) {
+        const pcl_material& m = mats[i];
+        if (m.kind < PCL_MAT_DIFFUSE || m.kind > PCL_MAT_DIELECTRIC) {
+            err = "material " + std::to_string(i) + " needs a feature outside the HIP core (principled BSDF, "
+                  "texture or dispersion)";
+            return PCL_ERR_UNSUPPORTED;
+        }
+        MatRec r{};
+        r.kind = m.kind;
+        for (int c = 0; c < 3; ++c) { r.albedo[c] = m.albedo[c]; r.spec_color[c] = m.specular_color[c]; r.emission[c] = m.emission[c]; }
+        r.refl = m.reflectivity; r.ior = m.ior; r.spec = m.specular; r.shin = m.shininess;
+        out.push_back(r);
+    }
+    return PCL_OK;
+}
+
 int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
     if (!d || d->n_prims < 0 || d->n_materials < 0 || d->n_lights < 0 ||
         (d->n_prims > 0 && !d->prims) || (d->n_materials > 0 && !d->materials) || (d->n_lights > 0 && !d->lights)) {
@@ -253,19 +271,9 @@ int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
         recs[i] = r;
         boxes[i] = b;
     }
-    out.mats.clear();
-    for (int i = 0; i < d->n_materials; ++i) {
-        const pcl_material& m = d->materials[i];
-        if (m.kind < PCL_MAT_DIFFUSE || m.kind > PCL_MAT_DIELECTRIC) {
-            err = "material " + std::to_string(i) + " needs a feature outside the HIP core (principled BSDF, "
-                  "texture or dispersion)";
-            return PCL_ERR_UNSUPPORTED;
-        }
-        MatRec r{};
-        r.kind = m.kind;
-        for (int c = 0; c < 3; ++c) { r.albedo[c] = m.albedo[c]; r.spec_color[c] = m.specular_color[c]; r.emission[c] = m.emission[c]; }
-        r.refl = m.reflectivity; r.ior = m.ior; r.spec = m.specular; r.shin = m.shininess;
-        out.mats.push_back(r);
+    {
+        const int rc = pack_materials(d->materials, d->n_materials, out.mats, err);
+        if (rc != PCL_OK) return rc;
     }
     out.lights.clear();
     for (int i = 0; i < d->n_lights; ++i) {

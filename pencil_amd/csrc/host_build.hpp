@@ -22,6 +22,9 @@ struct HostLayout {
     int32_t depth = 0;
 };
 
+// Device material records of a pcl_material table; PCL_OK or an error code.
+int pack_materials(const pcl_material* mats, int n, std::vector<MatRec>& out, std::string& err);
+
 // Returns PCL_OK or an error code with a message in err.
 int build_layout(const pcl_scene_desc* desc, HostLayout& out, std::string& err);
 

@@ -132,17 +132,20 @@ PCL_D f3 jitter_light(const f3& lpos, float radius, const f3& p, Rng& rng) {
     return lpos + (t1 * dx + t2 * dy) * radius;
 }
 
-// The light loop + ambient of Renderer::trace (renderer.hpp:514-577)
+// The light loop + ambient of Renderer::trace (renderer.hpp:514-577).  With
+// dq, also d local / d albedo (per channel): the lights' unoccluded I*ndotl
+// sample means plus the ambient term.
 template <class LC>
 PCL_D f3 whitted_local(const DevScene& S, const RSettings& st, const Ray& r, const HitRec& rec, const f3& albedo,
                        const f3& emission, float spec, float shin, const f3& spec_color, Rng& rng, Stack& stk,
-                       LC& c) {
+                       LC& c, f3* dq = nullptr) {
     const f3 sn = rec.normal;
     f3 local = emission;
+    f3 q(0, 0, 0);
     for (int li = 0; li < S.n_lights; ++li) {
         const LightRec L = load_light(S.lights, li);
         const f3 lpos = ld3(L.pos), I = ld3(L.I);
-        f3 contrib(0, 0, 0);
+        f3 contrib(0, 0, 0), qc(0, 0, 0);
         const int nsamp = (L.radius > 0.0f) ? st.soft_samples : 1;
         for (int s = 0; s < nsamp; ++s) {
             f3 L_pos = lpos;
@@ -156,6 +159,7 @@ PCL_D f3 whitted_local(const DevScene& S, const RSettings& st, const Ray& r, con
             if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
             float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
             f3 col = albedo * I * ndotl;
+            if (dq) qc += I * ndotl;
             if (spec > 0.0f && ndotl > 0.0f) {
                 f3 V = -r.d.normalized();
                 f3 H = (Ld + V).normalized();
@@ -166,8 +170,10 @@ PCL_D f3 whitted_local(const DevScene& S, const RSettings& st, const Ray& r, con
             contrib += col;
         }
         local += contrib / (float)nsamp;
+        if (dq) q += qc / (float)nsamp;
     }
     local += albedo * ld3(S.amb);
+    if (dq) *dq = q + ld3(S.amb);
     return local;
 }
 
@@ -181,7 +187,8 @@ struct Frame {
 
 // Pushes the glass frame of trace_dielectric (renderer.hpp:1582-1627) and
 // returns the reflection ray; the refraction ray is kept in the frame.
-PCL_D Ray push_glass(Frame* F, int& nf, const Ray& r, const HitRec& rec, const f3& albedo, float ior, int depth,
+template <class FR>
+PCL_D Ray push_glass(FR* F, int& nf, const Ray& r, const HitRec& rec, const f3& albedo, float ior, int depth,
                      float bias) {
     const bool entering = rec.front;
     const float safe = fmaxf(ior, 0.01f);
@@ -193,7 +200,7 @@ PCL_D Ray push_glass(Frame* F, int& nf, const Ray& r, const HitRec& rec, const f
     Ray rr;
     rr.o = rec.p + rec.normal * bias;
     rr.d = dn.reflect(rec.normal).normalized();
-    Frame& f = F[nf++];
+    FR& f = F[nf++];
     f.albedo = albedo;
     f.depth = depth;
     if (sin_t2 > 1.0f) {

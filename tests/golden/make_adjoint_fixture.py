@@ -1,0 +1,98 @@
+"""Generate the adjoint fixture tests/golden/adj_cornell.npz (run where /root/reference exists).
+
+Scene: pencil_amd/scenes/cornell_c5.pecl (8 materials).  The target images are
+rendered from its parameters; the loss and its gradient are evaluated at a
+seeded initial point, every material's albedo and emission drawn from
+U[0.2, 0.8], written as tests/golden/adj_init.pecl (colours are parsed as
+text / 100, so the .pecl carries the exact values both renderers see).
+
+For each mode (path tracing with 4 bounces -- Russian roulette never fires --
+and Whitted with depth 4) the reference renderer, through
+oracle/_ref/ref_harness, gives:
+  target_<mode>   HDR target image (true parameters)
+  loss_<mode>     sum (I - target)^2 at the initial point
+  fd_<mode>       central differences of that loss, h = 1e-3, over
+                  [m][albedo r g b, emission r g b]
+
+Usage: python tests/golden/make_adjoint_fixture.py
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+SCENE = os.path.join(ROOT, "pencil_amd", "scenes", "cornell_c5.pecl")
+INIT = os.path.join(HERE, "adj_init.pecl")
+W = H = 24
+MODES = {
+    "pt": ["--width", str(W), "--height", str(H), "--samples", "2", "--pathtrace", "--bounces", "4"],
+    "wh": ["--width", str(W), "--height", str(H), "--samples", "2", "--depth", "4"],
+}
+SEED = 5
+
+
+def run(args):
+    r = subprocess.run([HARNESS] + args, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness {args} failed: {r.stderr}")
+
+
+def write_init(params):
+    """cornell_c5.pecl with every mat{} block's clr/emt replaced by params."""
+    text = open(SCENE).read()
+    k = [0]
+
+    def sub(m):
+        a, e = params[k[0], :3] * 100.0, params[k[0], 3:] * 100.0
+        k[0] += 1
+        body = m.group(1)
+        body = re.sub(r"clr:\s*<[^>]*>", "clr: <%.6g, %.6g, %.6g>" % tuple(a), body)
+        body = re.sub(r",?\s*emt:\s*<[^>]*>", "", body)
+        return "mat {%s, emt: <%.6g, %.6g, %.6g> }" % ((body.rstrip(),) + tuple(e))
+
+    out = re.sub(r"mat \{([^}]*)\}", sub, text)
+    assert k[0] == params.shape[0]
+    with open(INIT, "w") as f:
+        f.write("# adj_init.pecl - cornell_c5.pecl at the seeded initial point of the adjoint fixture\n"
+                "# (written by make_adjoint_fixture.py)\n")
+        f.write(out)
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit(f"{HARNESS} missing: make -C oracle ref")
+    n_mat = len(re.findall(r"mat \{", open(SCENE).read()))
+    rng = np.random.default_rng(SEED)
+    params = rng.uniform(0.2, 0.8, size=(n_mat, 6))
+    write_init(params)
+    tmp = tempfile.mkdtemp()
+    fix = {"w": np.int32(W), "h": np.int32(H), "modes": json.dumps(MODES), "seed": np.int32(SEED)}
+    for mode, opts in MODES.items():
+        tgt = os.path.join(tmp, mode + ".bin")
+        run(["render", SCENE, tgt] + opts)
+        raw = open(tgt, "rb").read()
+        img = np.frombuffer(raw[8:8 + W * H * 12], dtype=np.float32).reshape(H, W, 3).copy()
+        tpath = os.path.join(tmp, mode + ".target")
+        img.tofile(tpath)
+        out = os.path.join(tmp, mode + ".loss")
+        run(["loss", INIT, tpath, out] + opts + ["--h", "0.001"])
+        vals = np.fromfile(out, dtype=np.float64)
+        fix["target_" + mode] = img
+        fix["loss_" + mode] = np.float64(vals[0])
+        fix["fd_" + mode] = vals[1:].reshape(n_mat, 6)
+        print(mode, "loss", vals[0], "fd norm", np.linalg.norm(vals[1:]))
+    np.savez_compressed(os.path.join(HERE, "adj_cornell.npz"), **fix)
+    print("done")
+
+
+if __name__ == "__main__":
+    main()
