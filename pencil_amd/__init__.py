@@ -166,6 +166,9 @@ def host_lib():
         _sig(lib, "pclh_make_stress", _P, [C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int])
         _sig(lib, "pclh_write_stress", C.c_int,
              [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_char_p, C.c_int])
+        _sig(lib, "pclh_make_cornell_mesh", _P, [C.c_char_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                                  C.c_float, C.c_int, C.c_int, C.c_char_p, C.c_int])
+        _sig(lib, "pclh_make_mixed", _P, [C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int])
         _sig(lib, "pclh_free", None, [_P])
         _sig(lib, "pclh_desc", C.c_int, [_P, C.POINTER(SceneDesc)])
         _sig(lib, "pclh_camera", C.c_int, [_P, C.POINTER(Camera)])
@@ -201,6 +204,21 @@ class HostScene:
     @classmethod
     def stress(cls, freq=229, n_spheres=4096, seed=1234, width=1920, height=1080):
         return cls(host_lib().pclh_make_stress(freq, n_spheres, seed, width, height))
+
+    @classmethod
+    def cornell_mesh(cls, pecl: str, freq=229, seed=1234, scale=0.25, center=(0.0, 0.8, 0.3), width=1920,
+                     height=1080):
+        """A Cornell-box .pecl with the displaced icosphere inside (the path-tracing workload)."""
+        err = C.create_string_buffer(4096)
+        h = host_lib().pclh_make_cornell_mesh(pecl.encode(), freq, seed, scale, *center, width, height, err, len(err))
+        if not h:
+            raise RuntimeError(err.value.decode())
+        return cls(h)
+
+    @classmethod
+    def mixed(cls, freq, n_spheres, seed, width, height):
+        """Icosphere + seeded diffuse / metal / glass spheres (one batched-workload scene)."""
+        return cls(host_lib().pclh_make_mixed(freq, n_spheres, seed, width, height))
 
     @staticmethod
     def write_stress(directory: str, name: str, freq=229, n_spheres=4096, seed=1234, width=1920, height=1080):

@@ -71,6 +71,34 @@ int pclh_write_stress(const char* dir, const char* name, int freq, int n_spheres
     return 0;
 }
 
+// The path-tracing workload: a Cornell box .pecl with the displaced
+// icosphere (frequency freq, seed) placed inside at v * scale + (cx, cy, cz).
+void* pclh_make_cornell_mesh(const char* pecl, int freq, uint32_t seed, float scale, float cx, float cy, float cz,
+                             int width, int height, char* err, int errlen) {
+    void* hv = pclh_load_pecl(pecl, width, height, 0, 0, err, errlen);
+    if (!hv) return nullptr;
+    auto* h = static_cast<Handle*>(hv);
+    pencil::StressParams P;
+    P.freq = freq; P.seed = seed;
+    P.mesh_scale = scale; P.mesh_offset[0] = cx; P.mesh_offset[1] = cy; P.mesh_offset[2] = cz;
+    pencil::add_mesh(P, h->ps);
+    h->refresh();
+    return h;
+}
+
+// One scene of the batched workload: a displaced icosphere of frequency freq
+// (radius 3) and n_spheres seeded spheres with a diffuse / metal / glass mix,
+// the stress scene's camera and a soft point light.
+void* pclh_make_mixed(int freq, int n_spheres, uint32_t seed, int width, int height) {
+    auto h = std::make_unique<Handle>();
+    pencil::StressParams P;
+    P.freq = freq; P.n_spheres = n_spheres; P.seed = seed; P.width = width; P.height = height; P.mat_mix = 1;
+    pencil::make_stress_scene(P, h->ps);
+    h->ps.scene.lights_mut()[0].radius = 0.5f;
+    h->refresh();
+    return h.release();
+}
+
 void pclh_free(void* h) { delete static_cast<Handle*>(h); }
 
 // Pointers inside *out stay valid until pclh_free or the next material edit.

@@ -67,6 +67,17 @@ inline Material make_diffuse(Color albedo, float spec = 0.0f, float shininess = 
     return m;
 }
 
+inline Material make_metal(Color albedo, float reflectivity) {
+    Material m;
+    m.kind = MaterialKind::Metal; m.albedo = albedo; m.reflectivity = reflectivity;
+    return m;
+}
+inline Material make_dielectric(Color albedo, float ior) {
+    Material m;
+    m.kind = MaterialKind::Dielectric; m.albedo = albedo; m.ior = ior;
+    return m;
+}
+
 // src/light.hpp:15-46
 enum class LightKind : int { Point = 0, Directional = 1 };
 
@@ -127,6 +138,7 @@ public:
     size_t primitive_count() const { return primitives_.size(); }
     const std::vector<Material>& materials() const { return materials_; }
     std::vector<Material>& materials_mut() { return materials_; }
+    std::vector<Light>& lights_mut() { return lights_; }
     const std::vector<Light>& lights() const { return lights_; }
     const std::vector<pcl_prim>& primitives() const { return primitives_; }
     const Material& material(int id) const { return materials_[id]; }

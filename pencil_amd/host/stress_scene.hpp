@@ -32,6 +32,11 @@ struct StressParams {
     uint32_t seed = 1234;
     int width = 1920, height = 1080;
     float radius = 3.0f, amplitude = 0.1f;
+    // mesh placement, applied to the displaced vertices: v * mesh_scale + mesh_offset
+    double mesh_scale = 1.0, mesh_offset[3] = {0.0, 0.0, 0.0};
+    // sphere materials: 0 = all diffuse; 1 = a seeded mix of diffuse, metal
+    // (fuzz) and glass spheres
+    int mat_mix = 0;
 };
 
 namespace stress {
@@ -101,7 +106,9 @@ inline void icosphere(const StressParams& P, std::vector<Vec3>& verts, std::vect
         double nx = q.x / l, ny = q.y / l, nz = q.z / l;
         double r = P.radius + P.amplitude * fbm3(nx * 1.0, ny * 1.0, nz * 1.0, P.seed);
         int32_t id = (int32_t)verts.size();
-        verts.push_back({(float)(nx * r), (float)(ny * r), (float)(nz * r)});
+        const double s = P.mesh_scale;
+        verts.push_back({(float)(nx * r * s + P.mesh_offset[0]), (float)(ny * r * s + P.mesh_offset[1]),
+                         (float)(nz * r * s + P.mesh_offset[2])});
         index.emplace(key, id);
         return id;
     };
@@ -150,7 +157,8 @@ inline void icosphere(const StressParams& P, std::vector<Vec3>& verts, std::vect
     }
 }
 
-struct SphereSpec { Vec3 c; float r; int clr[3]; };
+// kind: 0 diffuse, 1 metal (fuz percent in `fuz`), 2 glass (ior 1.5)
+struct SphereSpec { Vec3 c; float r; int clr[3]; int kind = 0; int fuz = 0; };
 
 inline std::vector<SphereSpec> spheres(const StressParams& P) {
     uint64_t s = 0x5EED0000ull ^ P.seed;
@@ -160,12 +168,41 @@ inline std::vector<SphereSpec> spheres(const StressParams& P) {
         q.c = {(float)(uniform(s) * 20.0 - 10.0), (float)(uniform(s) * 20.0 - 10.0), (float)(uniform(s) * 20.0 - 10.0)};
         q.r = (float)(0.02 + uniform(s) * 0.18);
         for (int c = 0; c < 3; ++c) q.clr[c] = 20 + (int)(splitmix(s) % 71u);
+        if (P.mat_mix) {
+            const uint32_t u = (uint32_t)(splitmix(s) % 8u);
+            if (u < 2) { q.kind = 1; q.fuz = (int)(splitmix(s) % 30u); }
+            else if (u == 2) q.kind = 2;
+        }
         out.push_back(q);
     }
     return out;
 }
 
 }  // namespace stress
+
+// The seeded spheres with their materials; `mat { clr, fuz | ior }` as the
+// parser turns them into materials (src/pecl_parser.hpp:739-762).
+inline void add_spheres(const StressParams& P, ParsedScene& ps) {
+    for (const auto& s : stress::spheres(P)) {
+        const Color c{(float)s.clr[0] / 100.0f, (float)s.clr[1] / 100.0f, (float)s.clr[2] / 100.0f};
+        Material m = make_diffuse(c, 0.0f, 32.0f);
+        if (s.kind == 1) m = make_metal(c, 1.0f - (float)s.fuz / 100.0f);
+        else if (s.kind == 2) m = make_dielectric(c, 1.5f);
+        ps.scene.add_material(m);
+        ps.scene.add_primitive(make_sphere(s.c, s.r, ps.scene.material_count() - 1));
+    }
+}
+
+// Appends the displaced icosphere (one diffuse 0.7 material) to a scene.
+inline void add_mesh(const StressParams& P, ParsedScene& ps) {
+    std::vector<Vec3> verts;
+    std::vector<int32_t> tris;
+    stress::icosphere(P, verts, tris);
+    ps.scene.add_material(make_diffuse({0.7f, 0.7f, 0.7f}, 0.0f, 32.0f));
+    const int mat = ps.scene.material_count() - 1;
+    for (size_t t = 0; t + 2 < tris.size(); t += 3)
+        ps.scene.add_primitive(make_triangle(verts[tris[t]], verts[tris[t + 1]], verts[tris[t + 2]], mat));
+}
 
 // Builds the scene in memory with exactly the floats the reference gets from
 // the .pecl + .obj that write_stress_scene emits.
@@ -187,12 +224,7 @@ inline void make_stress_scene(const StressParams& P, ParsedScene& ps) {
         Point3 v2 = verts[tris[t + 2]] * scale + off + user;
         ps.scene.add_primitive(make_triangle(v0, v1, v2, 0));
     }
-    for (const auto& s : stress::spheres(P)) {
-        Material m = make_diffuse({(float)s.clr[0] / 100.0f, (float)s.clr[1] / 100.0f, (float)s.clr[2] / 100.0f},
-                                  0.0f, 32.0f);
-        ps.scene.add_material(m);
-        ps.scene.add_primitive(make_sphere(s.c, s.r, ps.scene.material_count() - 1));
-    }
+    add_spheres(P, ps);
     ps.scene.add_light(make_point_light({10.0f, 15.0f, -10.0f}, {1.0f, 1.0f, 1.0f}, 0.0f));
     ps.eye = {0.0f, 4.0f, -18.0f};
     ps.look = {0.0f, 0.0f, 0.0f};
