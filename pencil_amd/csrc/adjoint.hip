@@ -50,7 +50,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adjoint(DevScene S, CamRec C, RSe
         }
         const int n = st.samples;
         const f3 W = Wp / (float)(n * n);   // the image is the mean of n*n samples
-        AdjFrame F[kMaxFrames];
+        alignas(16) unsigned char fbuf[sizeof(AdjFrame) * kMaxFrames];   // no zeroing constructors
+        AdjFrame* F = reinterpret_cast<AdjFrame*>(fbuf);
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         stk.stride = kAdjBlock;

@@ -99,7 +99,10 @@ __global__ __launch_bounds__(kBlock, 3) void k_render(DevScene S, CamRec C, RSet
     const int j = ty * 8 + (lane >> 3);
     LaneCount<COUNT> c;
     if (i < st.width && j < st.height) {
-        Frame F[kMaxFrames];
+        // raw storage: a Frame array would run f3's zeroing constructors on
+        // all 16 frames of every lane (~770 B of scratch writes per pixel)
+        alignas(16) unsigned char fbuf[sizeof(Frame) * kMaxFrames];
+        Frame* F = reinterpret_cast<Frame*>(fbuf);
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         stk.stride = kBlock;
