@@ -263,6 +263,21 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
 int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best,
                     double* best_ms);
 
+/* ---- 8-bit output on the device ------------------------------------------
+ * color_to_rgb8 (src/image_io.hpp:46-59) with tonemap_color's ACES curve
+ * (src/tonemap.hpp:27-61): scale by 2^exposure, ACES, clamp to [0, 1], gamma
+ * 1/2.2, (uint8)(x * 255 + 0.5), per channel.  Bit-identical to the host
+ * encode: the gamma step uses a threshold table built with the host's powf. */
+/* d_hdr: n_px*3 floats (16-byte aligned), d_rgb8: n_px*3 bytes (4-byte aligned), device. */
+int pcl_tonemap_rgb8_device(pcl_ctx* ctx, const float* d_hdr, int64_t n_px, float exposure, uint8_t* d_rgb8);
+/* pcl_render followed by the device encode: only the 8-bit image (W*H*3 bytes,
+ * host) leaves the device -- what write_image (src/image_io.hpp:61-113) stores. */
+int pcl_render_rgb8(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                    float exposure, uint8_t* out_rgb8, pcl_stats* stats);
+/* The 256-entry threshold table (t[0] = -inf; t[k] = smallest v with encode(v) >= k).
+ * Host only; needs no device. */
+int pcl_tonemap_thresholds(float* t256);
+
 /* ---- differentiation: adjoint (path-replay backprop) ----------------------
  * Parameters are every material's albedo (RGB) and emission (RGB), laid out
  * grad[m*6 + 0..2] = d/d albedo_m, grad[m*6 + 3..5] = d/d emission_m, the

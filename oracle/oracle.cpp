@@ -957,6 +957,33 @@ int orc_tonemap_rgb8(const float* hdr, int64_t npx, float exposure, uint8_t* out
     return 0;
 }
 
+// Checks a gamma threshold table against color_to_rgb8's encode step
+// (image_io.hpp:53-55) on every float in [0, 1]: returns the number of v with
+// (uint8)(powf(v, 1/2.2f) * 255 + 0.5) != #{k >= 1 : t[k] <= v}.
+int64_t orc_check_thresholds(const float* t, int threads) {
+    const uint32_t end = 0x3f800000u + 1;   // bit patterns 0 .. 1.0f inclusive
+    std::atomic<int64_t> bad{0};
+    std::vector<std::thread> pool;
+    if (threads < 1) threads = 1;
+    for (int w = 0; w < threads; ++w) {
+        pool.emplace_back([&, w]() {
+            int64_t local = 0;
+            for (uint32_t u = (uint32_t)w; u < end; u += (uint32_t)threads) {
+                float v;
+                std::memcpy(&v, &u, 4);
+                const int e = (int)(uint8_t)(std::pow(v, 1.0f / 2.2f) * 255.0f + 0.5f);
+                int lo = 0;
+                for (int step = 128; step >= 1; step >>= 1)
+                    if (t[lo + step] <= v) lo += step;
+                if (lo != e) ++local;
+            }
+            bad += local;
+        });
+    }
+    for (auto& th : pool) th.join();
+    return bad.load();
+}
+
 uint32_t orc_rng_u32(uint32_t seed, int32_t k) {
     orc::Rng r(seed);
     uint32_t v = 0;

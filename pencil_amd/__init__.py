@@ -140,6 +140,10 @@ def hip_lib():
         _sig(lib, "pcl_wf_autotune", C.c_int,
              [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(C.c_double)])
         _sig(lib, "pcl_scene_set_materials", C.c_int, [_P, C.c_int32, C.POINTER(Material), C.c_int32])
+        _sig(lib, "pcl_tonemap_rgb8_device", C.c_int, [_P, _P, C.c_int64, C.c_float, _P])
+        _sig(lib, "pcl_render_rgb8", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), C.c_float, _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_tonemap_thresholds", C.c_int, [_P])
         _sig(lib, "pcl_render_vjp_device", C.c_int,
              [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, C.POINTER(Stats)])
         _sig(lib, "pcl_adjoint", C.c_int,
@@ -296,6 +300,14 @@ def tonemap_rgb8(hdr: np.ndarray, exposure: float = 0.0) -> np.ndarray:
     return out
 
 
+def tonemap_thresholds() -> np.ndarray:
+    """The device encode's 256-entry gamma threshold table (host only)."""
+    t = np.empty(256, dtype=np.float32)
+    if hip_lib().pcl_tonemap_thresholds(_ptr(t)) != PCL_OK:
+        raise RuntimeError("pcl_tonemap_thresholds failed")
+    return t
+
+
 def bvh_leaf_order_host(desc: SceneDesc):
     """The core's host BVH build alone (no device): (leaf order, SceneInfo)."""
     info = SceneInfo()
@@ -330,6 +342,17 @@ class Core:
         self._nmat = getattr(self, "_nmat", {})
         self._nmat[sid.value] = desc.n_materials
         return sid.value
+
+    def tonemap_device(self, d_hdr: int, n_px: int, d_rgb8: int, exposure: float = 0.0):
+        self._check(self.lib.pcl_tonemap_rgb8_device(self.ctx, C.c_void_p(d_hdr), n_px, exposure, C.c_void_p(d_rgb8)),
+                    "pcl_tonemap_rgb8_device")
+
+    def render_rgb8(self, sid: int, cam: Camera, settings: Settings, exposure: float = 0.0):
+        out = np.empty((settings.height, settings.width, 3), dtype=np.uint8)
+        st = Stats()
+        self._check(self.lib.pcl_render_rgb8(self.ctx, sid, C.byref(cam), C.byref(settings), exposure, _ptr(out),
+                                             C.byref(st)), "pcl_render_rgb8")
+        return out, st
 
     def set_materials(self, sid: int, mats, n: int):
         self._check(self.lib.pcl_scene_set_materials(self.ctx, sid, mats, n), "pcl_scene_set_materials")

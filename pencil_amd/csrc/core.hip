@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "adjoint_launch.hpp"
+#include "tonemap_launch.hpp"
 #include "host_build.hpp"
 #include "pencil_hip.h"
 #include "render.hpp"
@@ -142,6 +144,9 @@ struct pcl_ctx {
     float* d_img = nullptr;          // forward image / target staging
     float* d_tgt = nullptr;
     size_t img_cap = 0;
+    float* d_thr = nullptr;          // tone-map threshold table (256 floats)
+    uint8_t* d_rgb8 = nullptr;
+    size_t rgb8_cap = 0;
 };
 
 namespace {
@@ -235,6 +240,12 @@ int pcl_init(int device, pcl_ctx** out) {
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return PCL_ERR_HIP;
     if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return PCL_ERR_HIP;
     if (hipMalloc(&ctx->d_ctr, kCtrSlots * sizeof(unsigned long long)) != hipSuccess) return PCL_ERR_OOM;
+    {
+        float thr[256];
+        pcl::tonemap_thresholds(thr);
+        if (hipMalloc(&ctx->d_thr, sizeof(thr)) != hipSuccess) return PCL_ERR_OOM;
+        if (hipMemcpy(ctx->d_thr, thr, sizeof(thr), hipMemcpyHostToDevice) != hipSuccess) return PCL_ERR_HIP;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cus = prop.multiProcessorCount;
@@ -254,6 +265,8 @@ void pcl_destroy(pcl_ctx* ctx) {
     if (ctx->d_adj_out) (void)hipFree(ctx->d_adj_out);
     if (ctx->d_img) (void)hipFree(ctx->d_img);
     if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
+    if (ctx->d_thr) (void)hipFree(ctx->d_thr);
+    if (ctx->d_rgb8) (void)hipFree(ctx->d_rgb8);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -619,6 +632,63 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     return PCL_OK;
 }
 
+// Grows the context's HDR staging buffers (forward image, target) to n floats.
+static int img_buffers(pcl_ctx* ctx, size_t n) {
+    if (n <= ctx->img_cap) return PCL_OK;
+    if (ctx->d_img) (void)hipFree(ctx->d_img);
+    if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
+    ctx->d_img = ctx->d_tgt = nullptr;
+    ctx->img_cap = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->d_img, n * sizeof(float)));
+    HIP_TRY(ctx, hipMalloc(&ctx->d_tgt, n * sizeof(float)));
+    ctx->img_cap = n;
+    return PCL_OK;
+}
+
+int pcl_tonemap_thresholds(float* t256) {
+    if (!t256) return PCL_ERR_INVALID;
+    pcl::tonemap_thresholds(t256);
+    return PCL_OK;
+}
+
+int pcl_tonemap_rgb8_device(pcl_ctx* ctx, const float* d_hdr, int64_t n_px, float exposure, uint8_t* d_rgb8) {
+    if (!ctx) return PCL_ERR_INVALID;
+    if (n_px < 0 || (n_px > 0 && (!d_hdr || !d_rgb8))) return fail(ctx, PCL_ERR_INVALID, "bad tone-map buffers");
+    if (((uintptr_t)d_hdr & 15) || ((uintptr_t)d_rgb8 & 3))
+        return fail(ctx, PCL_ERR_INVALID, "tone map: HDR buffer must be 16-byte and output 4-byte aligned");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const float scale = std::pow(2.0f, exposure);   // tonemap_color's e, with the host's libm
+    HIP_TRY(ctx, pcl::tm::launch(ctx->stream, d_hdr, n_px, scale, ctx->d_thr, d_rgb8));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return PCL_OK;
+}
+
+int pcl_render_rgb8(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                    float exposure, uint8_t* out_rgb8, pcl_stats* stats) {
+    if (!out_rgb8) return fail(ctx, PCL_ERR_INVALID, "null output");
+    int rc = check_settings(ctx, settings);
+    if (rc) return rc;
+    if (!get_scene(ctx, scene)) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
+    const double t0 = now_ms();
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n_px = (size_t)settings->width * settings->height;
+    if ((rc = img_buffers(ctx, n_px * 3))) return rc;
+    if (n_px * 3 > ctx->rgb8_cap) {
+        if (ctx->d_rgb8) (void)hipFree(ctx->d_rgb8);
+        ctx->d_rgb8 = nullptr;
+        ctx->rgb8_cap = 0;
+        HIP_TRY(ctx, hipMalloc(&ctx->d_rgb8, n_px * 3 + 4));
+        ctx->rgb8_cap = n_px * 3;
+    }
+    rc = render_impl(ctx, scene, cam, settings, ctx->d_img, stats, nullptr);
+    if (rc) return rc;
+    rc = pcl_tonemap_rgb8_device(ctx, ctx->d_img, (int64_t)n_px, exposure, ctx->d_rgb8);
+    if (rc) return rc;
+    HIP_TRY(ctx, hipMemcpy(out_rgb8, ctx->d_rgb8, n_px * 3, hipMemcpyDeviceToHost));
+    if (stats) stats->total_ms = now_ms() - t0;
+    return PCL_OK;
+}
+
 int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* materials, int32_t n) {
     SceneSlot* sl = get_scene(ctx, scene);
     if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
@@ -714,16 +784,7 @@ int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     float* img = d_out_rgb;
     if (!img) {   // forward image into the context's staging buffer
-        const size_t n = (size_t)settings->width * settings->height * 3;
-        if (n > ctx->img_cap) {
-            if (ctx->d_img) (void)hipFree(ctx->d_img);
-            if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
-            ctx->d_img = ctx->d_tgt = nullptr;
-            ctx->img_cap = 0;
-            HIP_TRY(ctx, hipMalloc(&ctx->d_img, n * sizeof(float)));
-            HIP_TRY(ctx, hipMalloc(&ctx->d_tgt, n * sizeof(float)));
-            ctx->img_cap = n;
-        }
+        if ((rc = img_buffers(ctx, (size_t)settings->width * settings->height * 3))) return rc;
         img = ctx->d_img;
     }
     pcl_stats fst{};
@@ -749,15 +810,7 @@ int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_se
     const double t0 = now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const size_t n = (size_t)settings->width * settings->height * 3;
-    if (n > ctx->img_cap) {
-        if (ctx->d_img) (void)hipFree(ctx->d_img);
-        if (ctx->d_tgt) (void)hipFree(ctx->d_tgt);
-        ctx->d_img = ctx->d_tgt = nullptr;
-        ctx->img_cap = 0;
-        HIP_TRY(ctx, hipMalloc(&ctx->d_img, n * sizeof(float)));
-        HIP_TRY(ctx, hipMalloc(&ctx->d_tgt, n * sizeof(float)));
-        ctx->img_cap = n;
-    }
+    if ((rc = img_buffers(ctx, n))) return rc;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tgt, target_rgb, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     rc = pcl_adjoint_device(ctx, scene, cam, settings, ctx->d_tgt, grad, loss, ctx->d_img, stats);
     if (rc) return rc;

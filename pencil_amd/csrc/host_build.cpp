@@ -183,6 +183,21 @@ struct Builder {
 
 }  // namespace
 
+void tonemap_thresholds(float t[256]) {
+    auto enc = [](float v) { return (int)(uint8_t)(std::pow(v, 1.0f / 2.2f) * 255.0f + 0.5f); };
+    t[0] = -INFINITY;
+    for (int k = 1; k < 256; ++k) {
+        uint32_t lo = 0, hi = 0x3f800000u;   // bit patterns of [0, 1]; enc(1) = 255
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            float v;
+            std::memcpy(&v, &mid, 4);
+            if (enc(v) >= k) hi = mid; else lo = mid + 1;
+        }
+        std::memcpy(&t[k], &lo, 4);
+    }
+}
+
 int pack_materials(const pcl_material* mats, int n, std::vector<MatRec>& out, std::string& err) {
     out.clear();
     for (int i = 0; i < n; ++i) {

@@ -22,6 +22,11 @@ struct HostLayout {
     int32_t depth = 0;
 };
 
+// t[k], k = 1..255: the smallest float v in [0, 1] whose 8-bit encode
+// (uint8)(powf(v, 1/2.2f) * 255 + 0.5) (src/image_io.hpp:53-55) is >= k,
+// found with this host's powf; t[0] = -inf.
+void tonemap_thresholds(float t[256]);
+
 // Device material records of a pcl_material table; PCL_OK or an error code.
 int pack_materials(const pcl_material* mats, int n, std::vector<MatRec>& out, std::string& err);
 

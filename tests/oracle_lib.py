@@ -45,6 +45,7 @@ def lib():
                                      C.c_int32, C.c_int32, C.c_int32]),
             ("orc_tonemap_rgb8", C.c_int, [P, C.c_int64, C.c_float, P]),
             ("orc_rng_u32", C.c_uint32, [C.c_uint32, C.c_int32]),
+            ("orc_check_thresholds", C.c_int64, [P, C.c_int32]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -111,6 +112,12 @@ def camera_rays(cam, W, H):
     out = np.empty(W * H, dtype=pa.RAY_DTYPE)
     lib().orc_camera_rays(C.byref(cam), W, H, _p(out))
     return out
+
+
+def check_thresholds(t, threads=os.cpu_count() or 1):
+    """Mismatches of a gamma threshold table against the encode on every float in [0, 1]."""
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    return int(lib().orc_check_thresholds(_p(t), threads))
 
 
 def tonemap_rgb8(hdr, exposure=0.0):

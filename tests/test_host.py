@@ -145,3 +145,34 @@ def test_cli_refuses_cpu_backend_and_unknown_features():
     assert r.returncode == 2 and "reference" in r.stderr
     r = subprocess.run([pa.CLI, golden("three_spheres.pecl"), "/tmp/x.ppm", "--bdpt"], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+# ---- device tone map: the threshold table the encode uses ------------------
+
+def _encode_with_thresholds(hdr, exposure, t):
+    """The device algorithm in numpy float32: scale, ACES, clamp, count of t[1..] <= m."""
+    x = hdr.astype(np.float32) * np.float32(2.0 ** exposure)
+    a, b, c, d, e = (np.float32(v) for v in (2.51, 0.03, 2.43, 0.59, 0.14))
+    with np.errstate(all="ignore"):
+        y = (x * (a * x + b)) / (x * (c * x + d) + e)
+    y = np.where(np.float32(0.0) < y, y, np.float32(0.0))
+    m = np.clip(y, np.float32(0.0), np.float32(1.0))
+    return np.searchsorted(t[1:], m, side="right").astype(np.uint8)
+
+
+def test_tonemap_thresholds_exhaustive():
+    """Every float in [0, 1] (1,065,353,217 of them) encodes through the
+    table exactly as color_to_rgb8's powf step does."""
+    t = pa.tonemap_thresholds()
+    assert t[0] == -np.inf and np.all(np.diff(t[1:]) > 0)
+    assert ol.check_thresholds(t) == 0
+
+
+def test_threshold_encode_matches_host_tonemap():
+    t = pa.tonemap_thresholds()
+    rng = np.random.default_rng(11)
+    vals = np.concatenate([rng.exponential(0.5, 300000), rng.uniform(0, 40, 100000),
+                           [0.0, -0.0, -1.0, 1e-30, 1e30, np.inf, -np.inf, np.nan]]).astype(np.float32)
+    vals = vals[: len(vals) // 3 * 3].reshape(-1, 3)
+    for exposure in (0.0, 1.5, -2.0):
+        assert np.array_equal(_encode_with_thresholds(vals, exposure, t), pa.tonemap_rgb8(vals, exposure))
