@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 2
+#define PCL_ABI_VERSION 3
 
 enum {
     PCL_OK              = 0,
@@ -126,7 +126,8 @@ typedef struct pcl_settings {
     int32_t max_depth;          /* Whitted recursion depth */
     float   shadow_bias;        /* default 1e-3 */
     int32_t soft_shadow_samples;/* default 4 */
-    int32_t pathtrace;          /* 0 = Whitted trace(), 1 = trace_path() */
+    int32_t pathtrace;          /* 0 = Whitted trace(), 1 = trace_path(), 2 = trace_path_mis()
+                                   (settings.mis, src/renderer.hpp:1205-1335) */
     int32_t pathtrace_bounces;  /* default 4 */
 } pcl_settings;
 

@@ -680,8 +680,9 @@ struct Renderer {
         return g * albedo;
     }
 
-    // Renderer::trace_path, renderer.hpp:829-972
-    V3 trace_path(const Ray& r0, int bounces, Rng& rng) const {
+    // Renderer::trace_path, renderer.hpp:829-972; with mis, trace_path_mis
+    // (renderer.hpp:1205-1335), which differs only in the light loop
+    V3 trace_path(const Ray& r0, int bounces, Rng& rng, bool mis = false) const {
         rays++;
         V3 T{1, 1, 1}, R{0, 0, 0};
         Ray cur = r0;
@@ -712,7 +713,28 @@ struct Renderer {
                 Ray sr{rec.p + rec.normal * st.shadow_bias, Ld};
                 if (!L.emissive && occl(sr, st.shadow_bias, dist - st.shadow_bias)) continue;
                 float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
-                R += T * albedo * L.I * ndotl;
+                if (!mis) {
+                    R += T * albedo * L.I * ndotl;
+                    continue;
+                }
+                // renderer.hpp:1264-1302: power-heuristic weights of the light
+                // sample against a cosine-hemisphere BRDF sample
+                if (ndotl <= 0.0f) continue;
+                float pdf_brdf = ndotl / 3.14159265f;
+                float pdf_light;
+                if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) {
+                    float rr = L.radius;
+                    float cos_at_light = (-Ld).dot(V3{0, 1, 0});
+                    if (cos_at_light < 0.01f) cos_at_light = 0.01f;
+                    pdf_light = (dist * dist) / (3.14159265f * rr * rr * cos_at_light);
+                } else {
+                    pdf_light = 1.0f;
+                }
+                float w_light = (pdf_light * pdf_light) / (pdf_light * pdf_light + pdf_brdf * pdf_brdf + 1e-9f);
+                float w_brdf = 1.0f - w_light;
+                V3 contrib_nee = albedo * L.I * ndotl;
+                V3 contrib_brdf = albedo * L.I;
+                R += T * (contrib_nee * w_light + contrib_brdf * w_brdf);
             }
             R += T * albedo * S.amb;
             if (bounces - depth > 3) {
@@ -765,7 +787,7 @@ struct Renderer {
                 uu += rng.sym() * (0.5f / (W * n));
                 vv += rng.sym() * (0.5f / (H * n));
                 Ray r = cam.ray_at(uu, vv, &rng);
-                if (st.pathtrace) acc += trace_path(r, st.pathtrace_bounces, rng);
+                if (st.pathtrace) acc += trace_path(r, st.pathtrace_bounces, rng, st.pathtrace == 2);
                 else acc += trace(r, st.max_depth, rng);
             }
         }

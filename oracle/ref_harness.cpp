@@ -19,7 +19,7 @@
 //   ref_harness bvh    <scene.pecl> <out.bin>          leaf order + split points
 //   ref_harness loss   <scene.pecl> <target.bin> <out.bin> [opts]
 //                       L2 loss and central FD of it over albedo/emission
-// opts: --width W --height H --samples n --depth d --pathtrace --bounces b
+// opts: --width W --height H --samples n --depth d --pathtrace --mis --bounces b
 //       --threads T --h step --no-fd
 
 #include <algorithm>
@@ -61,7 +61,7 @@ struct Tagged : Primitive {
 
 struct Opts {
     int width = 0, height = 0, samples = 0, depth = 0, threads = 0, bounces = 4;
-    bool pathtrace = false, fd = true;
+    bool pathtrace = false, fd = true, mis = false;
     float h = 1e-3f;
 };
 
@@ -77,6 +77,7 @@ static Opts parse_opts(int argc, char** argv, int from) {
         else if (s == "--threads") o.threads = atoi(nxt());
         else if (s == "--bounces") o.bounces = atoi(nxt());
         else if (s == "--pathtrace") o.pathtrace = true;
+        else if (s == "--mis") o.mis = true;
         else if (s == "--h") o.h = (float)atof(nxt());
         else if (s == "--no-fd") o.fd = false;
     }
@@ -108,6 +109,7 @@ static RenderSettings settings_for(const ParsedScene& ps, const Opts& o) {
     rs.threads = o.threads;
     rs.pathtrace = o.pathtrace || ps.enable_pathtrace;
     rs.pathtrace_bounces = o.bounces;
+    rs.mis = o.mis;
     return rs;
 }
 

@@ -202,6 +202,8 @@ int check_settings(pcl_ctx* ctx, const pcl_settings* s) {
     if (s->pathtrace && s->pathtrace_bounces > kMaxFrames)
         return fail(ctx, PCL_ERR_LIMIT, "settings: pathtrace_bounces above " + std::to_string(kMaxFrames));
     if (s->soft_shadow_samples < 0) return fail(ctx, PCL_ERR_INVALID, "settings: soft_shadow_samples < 0");
+    if (s->pathtrace < 0 || s->pathtrace > 2)
+        return fail(ctx, PCL_ERR_INVALID, "settings: pathtrace must be 0 (Whitted), 1 (path) or 2 (path + MIS)");
     return PCL_OK;
 }
 
@@ -727,6 +729,8 @@ static int adj_check(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const p
     if (!cam || !grad) return fail(ctx, PCL_ERR_INVALID, "null camera or gradient output");
     int rc = check_settings(ctx, settings);
     if (rc) return rc;
+    if (settings->pathtrace == 2)
+        return fail(ctx, PCL_ERR_UNSUPPORTED, "adjoint: the MIS path tracer has no adjoint here (pathtrace 0 or 1)");
     if (sl->n_mats > PCL_ADJ_MAX_MATERIALS)
         return fail(ctx, PCL_ERR_LIMIT, "adjoint: " + std::to_string(sl->n_mats) + " materials, at most " +
                                             std::to_string(PCL_ADJ_MAX_MATERIALS));

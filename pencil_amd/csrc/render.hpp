@@ -283,9 +283,33 @@ PCL_D f3 whitted(const DevScene& S, const RSettings& st, Ray r, int depth, Rng& 
     }
 }
 
-// Renderer::trace_path (renderer.hpp:829-972)
+// Power-heuristic light-sample term of trace_path_mis (renderer.hpp:1264-1302)
+// for an unoccluded sample with ndotl > 0: albedo * I * (ndotl * w_light)
+// + albedo * I * w_brdf, in the reference's operation order.
+PCL_D f3 mis_light_term(const LightRec& L, const f3& albedo, const f3& I, const f3& Ld, float dist, float ndotl) {
+    const float pdf_brdf = ndotl / 3.14159265f;
+    float pdf_light;
+    if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) {
+        const float rr = L.radius;
+        float cos_at_light = (-Ld).dot(f3(0, 1, 0));
+        if (cos_at_light < 0.01f) cos_at_light = 0.01f;
+        pdf_light = (dist * dist) / (3.14159265f * rr * rr * cos_at_light);
+    } else {
+        pdf_light = 1.0f;
+    }
+    const float w_light = (pdf_light * pdf_light) / (pdf_light * pdf_light + pdf_brdf * pdf_brdf + 1e-9f);
+    const float w_brdf = 1.0f - w_light;
+    const f3 contrib_nee = albedo * I * ndotl;
+    const f3 contrib_brdf = albedo * I;
+    return contrib_nee * w_light + contrib_brdf * w_brdf;
+}
+
+// Renderer::trace_path (renderer.hpp:829-972); with st.pathtrace == 2,
+// trace_path_mis (renderer.hpp:1205-1335), which differs only in the light
+// loop (its tangent frame is the diffuse branch's, computed earlier).
 template <class LC>
 PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, Frame* F, Stack& stk, LC& c) {
+    const bool mis = st.pathtrace == 2;
     c.rays++;
     const int bounces = st.bounces;
     f3 T(1, 1, 1), R(0, 0, 0);
@@ -313,7 +337,8 @@ PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, F
             sr.o = rec.p + rec.normal * st.shadow_bias; sr.d = Ld;
             if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
             float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
-            R += T * albedo * ld3(L.I) * ndotl;
+            if (!mis) R += T * albedo * ld3(L.I) * ndotl;
+            else if (!(ndotl <= 0.0f)) R += T * mis_light_term(L, albedo, ld3(L.I), Ld, dist, ndotl);
         }
         R += T * albedo * ld3(S.amb);
         if (bounces - depth > 3) {   // Russian roulette, :922-926

@@ -487,7 +487,11 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
                     }
                     const float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
                     f3 c;
-                    if (path) {
+                    int skip = 0;   // trace_path_mis adds nothing for ndotl <= 0
+                    if (path && st.pathtrace == 2) {
+                        if (!(ndotl <= 0.0f)) c = T * mis_light_term(L, albedo, I, Ld, dist, ndotl);
+                        else skip = 1 << 16;
+                    } else if (path) {
                         c = T * albedo * I * ndotl;                 // :914-915
                     } else {
                         c = albedo * I * ndotl;                     // :558-566
@@ -499,7 +503,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
                             c += spec_color * I * (sp * spec);
                         }
                     }
-                    SH[ns] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8)));
+                    SH[ns] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8) | skip));
                     OC[ns] = 0;
                     if (!L.emissive) {
                         C.q.sh_who[o] = make_int2(s, ns);
@@ -648,7 +652,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
                 const float4* SH = S.sh + (size_t)s * kMaxSh;
                 const uint8_t* OC = S.occl + (size_t)s * kMaxSh;
                 for (int q = 0; q < L.nsh; ++q)
-                    if (!OC[q]) L.R += f3(SH[q].x, SH[q].y, SH[q].z);
+                    if (!OC[q] && !(__float_as_int(SH[q].w) & (1 << 16))) L.R += f3(SH[q].x, SH[q].y, SH[q].z);
                 L.R += L.T * xyz(S.alb[s]) * amb;
                 if (L.outcome == V_P_END) {
                     status = finish_sample(C, L, L.R, rays);

@@ -122,7 +122,7 @@ int pclh_settings(void* hv, pcl_settings* out) {
     *out = pcl_settings{};
     out->width = ps.width; out->height = ps.height; out->samples = ps.samples; out->max_depth = ps.depth;
     out->shadow_bias = rs.shadow_bias; out->soft_shadow_samples = rs.soft_shadow_samples;
-    out->pathtrace = ps.enable_pathtrace ? 1 : 0; out->pathtrace_bounces = rs.pathtrace_bounces;
+    out->pathtrace = ps.enable_pathtrace ? (ps.enable_mis ? 2 : 1) : 0; out->pathtrace_bounces = rs.pathtrace_bounces;
     return 0;
 }
 

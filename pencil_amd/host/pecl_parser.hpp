@@ -40,6 +40,7 @@ struct ParsedScene {
     int samples = 4;
     int depth = 8;
     bool enable_pathtrace = false;
+    bool enable_mis = false;     // render { mis: true } (src/pecl_parser.hpp, rs.mis)
 
     // Features the reference supports that the HIP core does not render.
     std::vector<std::string> unsupported;
@@ -478,6 +479,7 @@ inline bool parse_pecl_source(const std::string& src, ParsedScene& out, std::vec
                                                          : (std::atof(v.text.c_str()) != 0.0);
                     consumed = (v.kind == TokKind::Ident || v.kind == TokKind::Number);
                     if (key == "pt" || key == "pathtrace") out.enable_pathtrace = on;
+                    else if (key == "mis") out.enable_mis = on;
                     else if (key == "bdpt" && (v.text == "uni" || v.text == "unidirectional")) out.enable_pathtrace = true;
                     else if (on) out.unsupported.push_back("render " + key);
                 }

@@ -1,7 +1,7 @@
 // pencil.cpp - command line front end, a drop-in for the reference CLI's
 // render path (src/pencil.cpp:186-269, 635-790) with one new flag:
 //   --backend hip   (the default here; `cpu` is the reference binary itself)
-// Flags the HIP core has no renderer for (--bdpt, --mis, --volume, fog,
+// Flags the HIP core has no renderer for (--bdpt, --volume, fog,
 // bloom, denoise, ...) are refused with exit code 2 instead of being ignored.
 #include "image_io.hpp"
 #include "pecl_parser.hpp"
@@ -31,6 +31,7 @@ void usage() {
                  "  --device N         HIP device\n"
                  "  --width N --height N --samples N --depth N --bounces N\n"
                  "  --pathtrace        path tracing with next-event estimation\n"
+                 "  --mis              with --pathtrace: MIS-weighted light samples (trace_path_mis)\n"
                  "  --no-bvh           linear primitive scan\n"
                  "  --exposure F --tonemap 0|1|2 --quiet\n";
 }
@@ -40,7 +41,7 @@ void usage() {
 int main(int argc, char** argv) {
     std::string scene_path, out_path, backend = "hip";
     int ow = 0, oh = 0, os = 0, od = 0, bounces = 4, device = 0, tonemap = 1;
-    bool quiet = false, pathtrace = false, no_bvh = false;
+    bool quiet = false, pathtrace = false, no_bvh = false, mis = false;
     float exposure = 0.0f;
     for (int i = 1; i < argc; ++i) {
         std::string s = argv[i];
@@ -58,6 +59,7 @@ int main(int argc, char** argv) {
         else if (s == "--bounces") bounces = std::atoi(next());
         else if (s == "--threads") next();   // host threads do not apply to the HIP core
         else if (s == "--pathtrace" || s == "--pt") pathtrace = true;
+        else if (s == "--mis") mis = true;
         else if (s == "--no-bvh" || s == "--nobvh") no_bvh = true;
         else if (s == "--quiet") quiet = true;
         else if (s == "--exposure") exposure = std::strtof(next(), nullptr);
@@ -99,6 +101,7 @@ int main(int argc, char** argv) {
     pencil::RenderSettings rs;
     rs.width = ps.width; rs.height = ps.height; rs.samples = ps.samples; rs.max_depth = ps.depth;
     rs.pathtrace = pathtrace || ps.enable_pathtrace;
+    rs.mis = mis || ps.enable_mis;   // src/pencil.cpp:749
     rs.pathtrace_bounces = bounces;
     rs.backend = pencil::Backend::Hip;
     rs.device = device;

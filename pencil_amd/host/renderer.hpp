@@ -30,6 +30,7 @@ struct RenderSettings {
     int threads = 0;
     int soft_shadow_samples = 4;
     bool pathtrace = false;
+    bool mis = false;            // trace_path_mis instead of trace_path (src/renderer.hpp:55)
     int pathtrace_bounces = 4;
     int tile_size = 64;
     Backend backend = Backend::Hip;
@@ -80,7 +81,8 @@ public:
         pcl_settings ps{};
         ps.width = W; ps.height = H; ps.samples = settings_.samples; ps.max_depth = settings_.max_depth;
         ps.shadow_bias = settings_.shadow_bias; ps.soft_shadow_samples = settings_.soft_shadow_samples;
-        ps.pathtrace = settings_.pathtrace ? 1 : 0; ps.pathtrace_bounces = settings_.pathtrace_bounces;
+        ps.pathtrace = settings_.pathtrace ? (settings_.mis ? 2 : 1) : 0;
+        ps.pathtrace_bounces = settings_.pathtrace_bounces;
         pcl_stats st{};
         static_assert(sizeof(Color) == 3 * sizeof(float), "Color is three packed floats");
         int rc = pcl_render(ctx_, scene_id_, &camera_.args(), &ps, reinterpret_cast<float*>(out_pixels.data()), &st);

@@ -15,7 +15,8 @@ own headers (oracle/Makefile target `ref`) -- and stores small .npz fixtures:
                        1,048,820-triangle stress scene
   meta.json            glibc version, sizes
 
-Usage: python tests/golden/make_fixtures.py [--skip-full]
+Usage: python tests/golden/make_fixtures.py [--skip-full] [--only NAME,NAME]
+(--only regenerates just those HDR fixtures and leaves everything else alone)
 """
 from __future__ import annotations
 
@@ -43,6 +44,10 @@ RENDERS = [(s, s, ["--width", "128", "--height", "96", "--samples", "2"]) for s 
     ("pt_cornell_rr", "cornell_box", ["--width", "64", "--height", "64", "--samples", "2", "--pathtrace", "--bounces", "8"]),
     ("pt_three_spheres", "three_spheres", ["--width", "96", "--height", "72", "--samples", "2", "--pathtrace", "--bounces", "6"]),
     ("pt_shapes", "shapes", ["--width", "96", "--height", "72", "--samples", "2", "--pathtrace", "--bounces", "5"]),
+    ("mis_cornell", "cornell_box", ["--width", "96", "--height", "96", "--samples", "2", "--pathtrace", "--mis",
+                                    "--bounces", "4"]),
+    ("mis_soft_shadows", "soft_shadows", ["--width", "96", "--height", "72", "--samples", "2", "--pathtrace", "--mis",
+                                          "--bounces", "6"]),
 ]
 
 
@@ -89,8 +94,16 @@ def main():
     if not os.path.exists(HARNESS):
         sys.exit(f"{HARNESS} missing: make -C oracle ref")
     tmp = tempfile.mkdtemp()
-    meta = {"glibc": platform.libc_ver()[1], "harness": "oracle/ref_harness.cpp", "renders": {}}
+    only = None
+    if "--only" in sys.argv:
+        only = set(sys.argv[sys.argv.index("--only") + 1].split(","))
+        with open(os.path.join(HERE, "meta.json")) as f:
+            meta = json.load(f)
+    else:
+        meta = {"glibc": platform.libc_ver()[1], "harness": "oracle/ref_harness.cpp", "renders": {}}
     for name, scene, opts in RENDERS:
+        if only is not None and name not in only:
+            continue
         out = os.path.join(tmp, name + ".bin")
         run(["render", os.path.join(HERE, scene + ".pecl"), out] + opts)
         img, rays = read_render(out)
@@ -98,6 +111,11 @@ def main():
                             scene=scene, opts=" ".join(opts))
         meta["renders"][name] = {"scene": scene, "opts": opts, "rays": rays}
         print("render", name, img.shape, rays)
+    if only is not None:
+        with open(os.path.join(HERE, "meta.json"), "w") as f:
+            json.dump(meta, f, indent=1)
+        print("done")
+        return
     for scene in SCENES:
         out = os.path.join(tmp, scene + ".hits")
         run(["hits", os.path.join(HERE, scene + ".pecl"), "160", "120", out])

@@ -206,3 +206,14 @@ def test_limits(core):
             core.set_materials(sid, mats, n - 1)
     finally:
         core.free_scene(sid)
+
+
+def test_mis_has_no_adjoint(core):
+    hs, st = scene("pt")
+    st.pathtrace = 2
+    sid = core.upload(hs.desc())
+    try:
+        with pytest.raises(pa.CoreError, match="MIS"):
+            core.adjoint(sid, hs.camera(), st, FIX["target_pt"])
+    finally:
+        core.free_scene(sid)

@@ -54,6 +54,7 @@ def _opts_to_kw(opts):
     if "--bounces" in vals:
         st["bounces"] = int(vals["--bounces"])
     st["pathtrace"] = "--pathtrace" in o
+    st["mis"] = "--mis" in o
     return kw, st
 
 
@@ -63,12 +64,13 @@ def render_fixture_inputs(fx_name):
     hs = load(str(fx["scene"]), **kw)
     s = hs.settings()
     if st["pathtrace"]:
-        s.pathtrace = 1
+        s.pathtrace = 2 if st["mis"] else 1
         s.pathtrace_bounces = st["bounces"]
     return fx, hs, s
 
 
-HDR_FIXTURES = SCENES + ["pt_cornell", "pt_cornell_rr", "pt_three_spheres", "pt_shapes"]
+HDR_FIXTURES = SCENES + ["pt_cornell", "pt_cornell_rr", "pt_three_spheres", "pt_shapes", "mis_cornell",
+                         "mis_soft_shadows"]
 
 
 @pytest.mark.parametrize("fx_name", HDR_FIXTURES)
