@@ -192,10 +192,29 @@ def leg_stress(core, pa, args):
         log(f"batched: {wms:.3f} ms/frame wall, {batched['kernel_ms']:.3f} ms on the stream, "
             f"{batched['mrays']:.1f} Mrays/s, RMSE vs plain {batched['rmse_vs_plain']:.3g}")
         core.free(d_wf)
+    # the device BVH builder (use_bvh = 2) on the same scene: build time and frame time
+    desc.use_bvh = 2
+    t0 = time.perf_counter()
+    sid2 = core.upload(desc)
+    up_ms = (time.perf_counter() - t0) * 1000.0
+    info2 = core.scene_info(sid2)
+    for _ in range(args.warmup):
+        core.render_device(sid2, cam, st, d_out)
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        s2 = core.render_device(sid2, cam, st, d_out)
+    lb_ms = (time.perf_counter() - t_start) * 1000.0 / args.steps
+    lbvh = {"upload_ms_total": up_ms, "host_prep_ms": info2.build_ms, "device_build_and_upload_ms": info2.upload_ms,
+            "depth": info2.depth, "ms_per_step": lb_ms,
+            "mrays": (s2.closest_queries + s2.anyhit_queries) / (lb_ms / 1000.0) / 1e6,
+            "host_sah_build_ms": info.build_ms}
+    log(f"device LBVH: upload {up_ms:.1f} ms (host prep {info2.build_ms:.1f}, device build + copy "
+        f"{info2.upload_ms:.1f}), depth {info2.depth}, {lb_ms:.3f} ms/frame")
+    core.free_scene(sid2)
     core.free(d_out)
     core.free_scene(sid)
     return dict(value=value, ms_per_step=ms_per_step, kernel_ms=kernel_ms, achieved=achieved, queries=queries,
-                cst=cst, info=info, desc_n=desc.n_prims, W=W, H=H, batched=batched)
+                cst=cst, info=info, desc_n=desc.n_prims, W=W, H=H, batched=batched, lbvh=lbvh)
 
 
 # ---------------------------------------------------------------------------
@@ -499,6 +518,7 @@ def main():
                    "path": "plain",
                    "plain_mrays_s": round(value, 3),
                    "batched": rounded(batched) if batched else None,
+                   "device_bvh_build": rounded(r["lbvh"]),
                    "adjoint_grad_steps_per_s": (round(legs["inverse"]["grad_steps_per_s"], 3)
                                                 if "inverse" in legs else None)},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

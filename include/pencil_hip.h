@@ -103,7 +103,11 @@ typedef struct pcl_scene_desc {
     int32_t             n_lights;
     float               background[3];
     float               ambient[3];
-    int32_t             use_bvh;    /* Scene::use_bvh (src/scene.hpp:53) */
+    int32_t             use_bvh;    /* Scene::use_bvh (src/scene.hpp:53): 0 linear scan, 1 BVH with the
+                                       reference's topology (host SAH build, BVH::build src/bvh.hpp:54-197),
+                                       2 BVH built on the device (Morton-ordered LBVH, milliseconds for 1M
+                                       primitives; another topology: the same hits except where equal-t
+                                       ties resolve by DFS order) */
 } pcl_scene_desc;
 
 /* Camera constructor arguments (src/camera.hpp:29-36, camera.cpp:48-68). */

@@ -21,6 +21,7 @@
 #include "adjoint_launch.hpp"
 #include "tonemap_launch.hpp"
 #include "host_build.hpp"
+#include "lbvh.hpp"
 #include "pencil_hip.h"
 #include "render.hpp"
 #include "wavefront.hpp"
@@ -277,6 +278,67 @@ void pcl_destroy(pcl_ctx* ctx) {
 
 const char* pcl_last_error(const pcl_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+// use_bvh == 2: the records go up in insertion order with their boxes, the
+// device builds the tree (lbvh.hip) and the leaf order, root box and depth
+// come back.  Work arrays are freed before returning.
+static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
+    const int n = (int)L.prims.size();
+    DevScene& D = sl.dev;
+    auto keep = [&](size_t bytes, void** p) -> int {
+        HIP_TRY(ctx, hipMalloc(p, bytes));
+        sl.bufs.push_back(*p);
+        sl.info.device_bytes += bytes;
+        return PCL_OK;
+    };
+    void *prims = nullptr, *nodes = nullptr, *slot_of = nullptr;
+    int rc;
+    if ((rc = keep(sizeof(PrimRec) * n, &prims)) || (rc = keep(sizeof(NodeRec) * (n - 1), &nodes)) ||
+        (rc = keep(sizeof(int32_t) * n, &slot_of)))
+        return rc;
+    std::vector<void*> tmp;
+    auto scratch = [&](size_t bytes, void** p) -> int {
+        HIP_TRY(ctx, hipMalloc(p, bytes < 256 ? 256 : bytes));
+        tmp.push_back(*p);
+        return PCL_OK;
+    };
+    auto free_tmp = [&]() { for (void* p : tmp) (void)hipFree(p); };
+    void *in = nullptr, *boxes = nullptr, *order = nullptr, *ka = nullptr, *kb = nullptr, *pi = nullptr,
+         *pl = nullptr, *flags = nullptr, *height = nullptr, *out7 = nullptr, *work = nullptr;
+    const size_t work_bytes = pcl::lbvh::temp_bytes(n);
+    if ((rc = scratch(sizeof(PrimRec) * n, &in)) || (rc = scratch(sizeof(float) * 6 * n, &boxes)) ||
+        (rc = scratch(sizeof(int32_t) * n, &order)) || (rc = scratch(sizeof(uint64_t) * n, &ka)) ||
+        (rc = scratch(sizeof(uint64_t) * n, &kb)) || (rc = scratch(sizeof(int32_t) * n, &pi)) ||
+        (rc = scratch(sizeof(int32_t) * n, &pl)) || (rc = scratch(sizeof(uint32_t) * n, &flags)) ||
+        (rc = scratch(sizeof(int32_t) * 2 * n, &height)) || (rc = scratch(sizeof(float) * 16, &out7)) ||
+        (rc = scratch(work_bytes, &work))) {
+        free_tmp();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(in, L.prims.data(), sizeof(PrimRec) * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(boxes, L.boxes.data(), sizeof(float) * 6 * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+        e = pcl::lbvh::build(ctx->stream, (const PrimRec*)in, (const float*)boxes, n, (PrimRec*)prims,
+                             (NodeRec*)nodes, (int32_t*)order, (int32_t*)slot_of, work, work_bytes, (uint64_t*)ka,
+                             (uint64_t*)kb, (int32_t*)pi, (int32_t*)pl, (uint32_t*)flags, (int32_t*)height,
+                             (float*)out7);
+    float h7[8] = {0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h7, out7, sizeof(h7), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(L.leaf_order.data(), order, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    free_tmp();
+    if (e != hipSuccess) return fail(ctx, PCL_ERR_HIP, std::string("device BVH build: ") + hipGetErrorString(e));
+    for (int k = 0; k < 6; ++k) L.root_box[k] = h7[k];
+    std::memcpy(&L.depth, &h7[6], sizeof(int32_t));
+    if (L.depth <= 0) return fail(ctx, PCL_ERR_HIP, "device BVH build produced no root");
+    L.root = 0;
+    D.prims = (const PrimRec*)prims;
+    D.nodes = (const NodeRec*)nodes;
+    D.slot_of = (const int32_t*)slot_of;
+    return PCL_OK;
+}
+
 int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_out) {
     if (!ctx || !desc || !scene_out) return fail(ctx, PCL_ERR_INVALID, "null argument");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -285,9 +347,6 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     std::string err;
     int rc = build_layout(desc, L, err);
     if (rc != PCL_OK) return fail(ctx, rc, err);
-    if (L.depth > kStackK + kStackSpill)
-        return fail(ctx, PCL_ERR_LIMIT, "BVH depth " + std::to_string(L.depth) + " exceeds the traversal stack (" +
-                                            std::to_string(kStackK + kStackSpill) + ")");
     const double t1 = now_ms();
     int32_t idx = -1;
     for (size_t i = 0; i < ctx->scenes.size(); ++i)
@@ -298,25 +357,37 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     sl.used = true;
     DevScene& D = sl.dev;
     int r = PCL_OK;
-    if ((r = upload(ctx, sl, L.nodes, &D.nodes)) || (r = upload(ctx, sl, L.prims, &D.prims)) ||
-        (r = upload(ctx, sl, L.mats, &D.mats)) || (r = upload(ctx, sl, L.lights, &D.lights)) ||
-        (r = upload(ctx, sl, L.slot_of, &D.slot_of))) {
+    const bool device_build = desc->use_bvh == 2 && desc->n_prims > 1;
+    if (device_build) {
+        if ((r = build_on_device(ctx, sl, L)) || (r = upload(ctx, sl, L.mats, &D.mats)) ||
+            (r = upload(ctx, sl, L.lights, &D.lights))) {
+            release(sl);
+            return r;
+        }
+    } else if ((r = upload(ctx, sl, L.nodes, &D.nodes)) || (r = upload(ctx, sl, L.prims, &D.prims)) ||
+               (r = upload(ctx, sl, L.mats, &D.mats)) || (r = upload(ctx, sl, L.lights, &D.lights)) ||
+               (r = upload(ctx, sl, L.slot_of, &D.slot_of))) {
         release(sl);
         return r;
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) { release(sl); return fail(ctx, PCL_ERR_HIP, hipGetErrorString(e)); }
+    if (L.depth > kStackK + kStackSpill) {
+        release(sl);
+        return fail(ctx, PCL_ERR_LIMIT, "BVH depth " + std::to_string(L.depth) + " exceeds the traversal stack (" +
+                                            std::to_string(kStackK + kStackSpill) + ")");
+    }
     D.n_prims = desc->n_prims;
     D.n_lights = desc->n_lights;
     D.root = L.root;
-    D.use_bvh = desc->use_bvh ? 1 : 0;
+    D.use_bvh = desc->use_bvh ? 1 : 0;   // 1 and 2 both leave a BVH on the device
     for (int k = 0; k < 6; ++k) D.root_box[k] = L.root_box[k];
     for (int k = 0; k < 3; ++k) { D.bg[k] = desc->background[k]; D.amb[k] = desc->ambient[k]; }
     sl.leaf_order = std::move(L.leaf_order);
     sl.lights = L.lights;
     sl.n_mats = desc->n_materials;
     sl.info.n_prims = desc->n_prims;
-    sl.info.n_nodes = (int32_t)L.nodes.size();
+    sl.info.n_nodes = device_build ? desc->n_prims - 1 : (int32_t)L.nodes.size();
     sl.info.depth = L.depth;
     sl.info.root = L.root;
     sl.info.build_ms = t1 - t0;

@@ -304,6 +304,26 @@ int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
         out.lights.push_back(r);
     }
 
+    if (d->use_bvh == 2) {   // the device builds the tree (lbvh.hip)
+        out.nodes.clear();
+        out.boxes.resize((size_t)n * 6);
+        for (int i = 0; i < n; ++i) {
+            put(&out.boxes[(size_t)i * 6], boxes[i].mn);
+            put(&out.boxes[(size_t)i * 6 + 3], boxes[i].mx);
+        }
+        out.prims = std::move(recs);
+        out.leaf_order.resize((size_t)n);
+        out.slot_of.resize((size_t)n);
+        for (int i = 0; i < n; ++i) out.leaf_order[i] = out.slot_of[i] = i;
+        out.root = n > 0 ? (n == 1 ? ~0 : 0) : kEmptyRoot;
+        if (n > 0) { put(out.root_box, boxes[0].mn); put(out.root_box + 3, boxes[0].mx); }
+        out.depth = 0;
+        return PCL_OK;
+    }
+    if (d->use_bvh != 0 && d->use_bvh != 1) {
+        err = "use_bvh must be 0 (linear scan), 1 (reference SAH build) or 2 (device LBVH build)";
+        return PCL_ERR_INVALID;
+    }
     Builder B;
     B.nodes = &out.nodes;
     B.storage.resize((size_t)n);

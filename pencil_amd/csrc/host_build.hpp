@@ -20,6 +20,9 @@ struct HostLayout {
     int32_t root = kEmptyRoot;
     float root_box[6] = {0, 0, 0, 0, 0, 0};
     int32_t depth = 0;
+    // use_bvh == 2 (device builder): prims stay in insertion order, no nodes
+    // are built here, and boxes (6 floats per primitive) go to the device
+    std::vector<float> boxes;
 };
 
 // t[k], k = 1..255: the smallest float v in [0, 1] whose 8-bit encode
