@@ -303,13 +303,13 @@ static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
     };
     auto free_tmp = [&]() { for (void* p : tmp) (void)hipFree(p); };
     void *in = nullptr, *boxes = nullptr, *order = nullptr, *ka = nullptr, *kb = nullptr, *pi = nullptr,
-         *pl = nullptr, *flags = nullptr, *height = nullptr, *out7 = nullptr, *work = nullptr;
+         *pl = nullptr, *nd = nullptr, *sorted = nullptr, *out16 = nullptr, *work = nullptr;
     const size_t work_bytes = pcl::lbvh::temp_bytes(n);
     if ((rc = scratch(sizeof(PrimRec) * n, &in)) || (rc = scratch(sizeof(float) * 6 * n, &boxes)) ||
         (rc = scratch(sizeof(int32_t) * n, &order)) || (rc = scratch(sizeof(uint64_t) * n, &ka)) ||
         (rc = scratch(sizeof(uint64_t) * n, &kb)) || (rc = scratch(sizeof(int32_t) * n, &pi)) ||
-        (rc = scratch(sizeof(int32_t) * n, &pl)) || (rc = scratch(sizeof(uint32_t) * n, &flags)) ||
-        (rc = scratch(sizeof(int32_t) * 2 * n, &height)) || (rc = scratch(sizeof(float) * 16, &out7)) ||
+        (rc = scratch(sizeof(int32_t) * n, &pl)) || (rc = scratch(sizeof(int32_t) * n, &nd)) ||
+        (rc = scratch(sizeof(float) * 6 * n, &sorted)) || (rc = scratch(sizeof(float) * 16, &out16)) ||
         (rc = scratch(work_bytes, &work))) {
         free_tmp();
         return rc;
@@ -317,21 +317,21 @@ static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
     hipError_t e = hipMemcpyAsync(in, L.prims.data(), sizeof(PrimRec) * n, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(boxes, L.boxes.data(), sizeof(float) * 6 * n, hipMemcpyHostToDevice, ctx->stream);
+    int depth = 0;
     if (e == hipSuccess)
         e = pcl::lbvh::build(ctx->stream, (const PrimRec*)in, (const float*)boxes, n, (PrimRec*)prims,
                              (NodeRec*)nodes, (int32_t*)order, (int32_t*)slot_of, work, work_bytes, (uint64_t*)ka,
-                             (uint64_t*)kb, (int32_t*)pi, (int32_t*)pl, (uint32_t*)flags, (int32_t*)height,
-                             (float*)out7);
-    float h7[8] = {0};
-    if (e == hipSuccess) e = hipMemcpyAsync(h7, out7, sizeof(h7), hipMemcpyDeviceToHost, ctx->stream);
+                             (uint64_t*)kb, (int32_t*)pi, (int32_t*)pl, (int32_t*)nd, (float*)sorted,
+                             (float*)out16, &depth);
+    float h6[6] = {0};
+    if (e == hipSuccess) e = hipMemcpyAsync(h6, out16, sizeof(h6), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(L.leaf_order.data(), order, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     free_tmp();
     if (e != hipSuccess) return fail(ctx, PCL_ERR_HIP, std::string("device BVH build: ") + hipGetErrorString(e));
-    for (int k = 0; k < 6; ++k) L.root_box[k] = h7[k];
-    std::memcpy(&L.depth, &h7[6], sizeof(int32_t));
-    if (L.depth <= 0) return fail(ctx, PCL_ERR_HIP, "device BVH build produced no root");
+    for (int k = 0; k < 6; ++k) L.root_box[k] = h6[k];
+    L.depth = depth;
     L.root = 0;
     D.prims = (const PrimRec*)prims;
     D.nodes = (const NodeRec*)nodes;

@@ -118,71 +118,75 @@ __global__ __launch_bounds__(kBlock) void k_leaves(const uint64_t* k, int n, con
     slot_of[id] = s;
 }
 
-__device__ __forceinline__ void put_box(float* dst, const float* b) {
-    for (int a = 0; a < 6; ++a) dst[a] = b[a];
+// Depth of every internal node (root = 0) by walking its parent chain:
+// unique 64-bit keys fix at least one more key bit per level, so no chain is
+// longer than 64.
+__global__ __launch_bounds__(kBlock) void k_depth(int n, const int32_t* parent_int, int32_t* ndepth) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = 0, v = i;
+    while (v != 0 && d < 65) { v = parent_int[v]; ++d; }
+    ndepth[i] = d;
 }
 
-// Bottom-up boxes: each leaf climbs; at a node the first arrival stops, the
-// second (which sees the other child's box after the device-scope fence)
-// merges both and climbs on.  Heights give the depth (internal levels).
-__global__ __launch_bounds__(kBlock) void k_boxes(const uint64_t* k, int n, const float* boxes,
-                                                  NodeRec* nodes, const int32_t* parent_int,
-                                                  const int32_t* parent_leaf, uint32_t* flags, int32_t* height,
-                                                  float* root_box, int32_t* depth_out) {
+__device__ __forceinline__ void child_box(const NodeRec* nodes, const float* sorted_boxes, int32_t ref, float* b) {
+    if (ref < 0) {
+        const float* p = sorted_boxes + (size_t)(~ref) * 6;
+        for (int a = 0; a < 6; ++a) b[a] = p[a];
+    } else {
+        const NodeRec& c = nodes[ref];
+        for (int a = 0; a < 3; ++a) {
+            b[a] = fminf(c.lb[a], c.rb[a]);
+            b[3 + a] = fmaxf(c.lb[3 + a], c.rb[3 + a]);
+        }
+    }
+}
+
+// Boxes of the internal nodes at one depth, from their children (leaves, or
+// nodes one level deeper whose boxes the previous launch wrote).
+__global__ __launch_bounds__(kBlock) void k_level(int n, int depth, const int32_t* ndepth, NodeRec* nodes,
+                                                  const float* sorted_boxes) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1 || ndepth[i] != depth) return;
+    NodeRec& nd = nodes[i];
+    child_box(nodes, sorted_boxes, nd.left, nd.lb);
+    child_box(nodes, sorted_boxes, nd.right, nd.rb);
+}
+
+__global__ void k_root(const NodeRec* nodes, float* out7) {
+    if (threadIdx.x != 0) return;
+    const NodeRec& r = nodes[0];
+    for (int a = 0; a < 3; ++a) {
+        out7[a] = fminf(r.lb[a], r.rb[a]);
+        out7[3 + a] = fmaxf(r.lb[3 + a], r.rb[3 + a]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sort_boxes(const uint64_t* k, int n, const float* boxes, float* sorted) {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    float b[6];
-    const float* pb = boxes + (size_t)(uint32_t)k[s] * 6;
-    for (int a = 0; a < 6; ++a) b[a] = pb[a];
-    int32_t child = ~s;
-    int h = 0;
-    int node = parent_leaf[s];
-    // unique 64-bit keys: every level down fixes at least one more key bit,
-    // so no path is longer than 64 internal nodes
-    for (int guard = 0; guard < 65; ++guard) {
-        NodeRec& nd = nodes[node];
-        float* slot = (nd.left == child) ? nd.lb : nd.rb;
-        put_box(slot, b);
-        height[node * 2 + ((nd.left == child) ? 0 : 1)] = h;
-        __threadfence();
-        const uint32_t prev = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0) return;
-        __threadfence();
-        const volatile float* L = nd.lb;
-        const volatile float* R = nd.rb;
-        for (int a = 0; a < 3; ++a) {
-            b[a] = fminf(L[a], R[a]);
-            b[3 + a] = fmaxf(L[3 + a], R[3 + a]);
-        }
-        const volatile int32_t* hv = height;
-        const int hl = hv[node * 2], hr = hv[node * 2 + 1];
-        h = (hl > hr ? hl : hr) + 1;
-        if (node == 0) {
-            put_box(root_box, b);
-            *depth_out = h;
-            return;
-        }
-        child = node;
-        node = parent_int[node];
-    }
-    *depth_out = -1;   // unreachable for a well-formed tree; reported as an error
+    const float* p = boxes + (size_t)(uint32_t)k[s] * 6;
+    for (int a = 0; a < 6; ++a) sorted[(size_t)s * 6 + a] = p[a];
 }
 
 }  // namespace
 
 size_t temp_bytes(int n) {
-    size_t sort = 0;
-    hipcub::DeviceRadixSort::SortKeys(nullptr, sort, (const uint64_t*)nullptr, (uint64_t*)nullptr, n > 0 ? n : 1);
-    return sort + 256;
+    size_t sort = 0, red = 0;
+    const int m = n > 1 ? n : 2;
+    hipcub::DeviceRadixSort::SortKeys(nullptr, sort, (const uint64_t*)nullptr, (uint64_t*)nullptr, m);
+    hipcub::DeviceReduce::Max(nullptr, red, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
+    return (sort > red ? sort : red) + 256;
 }
 
 hipError_t build(hipStream_t st, const PrimRec* d_prims_in, const float* d_boxes, int n, PrimRec* d_prims_out,
                  NodeRec* d_nodes, int32_t* d_leaf_order, int32_t* d_slot_of, void* d_temp, size_t temp_size,
                  uint64_t* d_keys_a, uint64_t* d_keys_b, int32_t* d_parent_int, int32_t* d_parent_leaf,
-                 uint32_t* d_flags, int32_t* d_height, float* d_out7) {
+                 int32_t* d_ndepth, float* d_sorted_boxes, float* d_out16, int* depth) {
+    *depth = 0;
     if (n <= 0) return hipSuccess;
     const int gb = (n + kBlock - 1) / kBlock;
-    float* cb = d_out7 + 8;   // centers' bounds scratch after root box (6) + depth (1)
+    float* cb = d_out16 + 8;   // centers' bounds, after root box (6) and the depth slot
     hipLaunchKernelGGL(k_center_bounds, dim3(1), dim3(1024), 0, st, d_boxes, n, cb);
     hipLaunchKernelGGL(k_keys, dim3(gb), dim3(kBlock), 0, st, d_boxes, n, cb, d_keys_a);
     size_t tb = temp_size;
@@ -190,16 +194,26 @@ hipError_t build(hipStream_t st, const PrimRec* d_prims_in, const float* d_boxes
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_leaves, dim3(gb), dim3(kBlock), 0, st, d_keys_b, n, d_prims_in, d_prims_out, d_leaf_order,
                        d_slot_of);
-    if (n == 1) {
-        // a single leaf: no internal node; the caller takes the root box from the primitive
-        return hipGetLastError();
-    }
+    if (n == 1) return hipGetLastError();   // a single leaf: no internal node
     const int gi = (n - 1 + kBlock - 1) / kBlock;
     hipLaunchKernelGGL(k_tree, dim3(gi), dim3(kBlock), 0, st, d_keys_b, n, d_nodes, d_parent_int, d_parent_leaf);
-    e = hipMemsetAsync(d_flags, 0, sizeof(uint32_t) * (size_t)(n - 1), st);
+    hipLaunchKernelGGL(k_sort_boxes, dim3(gb), dim3(kBlock), 0, st, d_keys_b, n, d_boxes, d_sorted_boxes);
+    hipLaunchKernelGGL(k_depth, dim3(gi), dim3(kBlock), 0, st, n, d_parent_int, d_ndepth);
+    int32_t* d_max = reinterpret_cast<int32_t*>(d_out16 + 6);
+    tb = temp_size;
+    e = hipcub::DeviceReduce::Max(d_temp, tb, d_ndepth, d_max, n - 1, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_boxes, dim3(gb), dim3(kBlock), 0, st, d_keys_b, n, d_boxes, d_nodes, d_parent_int,
-                       d_parent_leaf, d_flags, d_height, d_out7, reinterpret_cast<int32_t*>(d_out7 + 6));
+    int32_t maxd = 0;
+    e = hipMemcpyAsync(&maxd, d_max, sizeof(maxd), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (maxd < 0 || maxd > 64) return hipErrorUnknown;   // impossible for unique keys
+    // boxes bottom-up, one depth per launch: kernel order makes the children's
+    // boxes visible to their parents, no atomics or fences needed
+    for (int d = maxd; d >= 0; --d)
+        hipLaunchKernelGGL(k_level, dim3(gi), dim3(kBlock), 0, st, n, d, d_ndepth, d_nodes, d_sorted_boxes);
+    hipLaunchKernelGGL(k_root, dim3(1), dim3(64), 0, st, d_nodes, d_out16);
+    *depth = maxd + 1;   // internal nodes on the longest root-to-leaf path
     return hipGetLastError();
 }
 
