@@ -594,10 +594,32 @@ struct Renderer {
     pcl_settings st;
     mutable std::atomic<uint64_t> rays{0};
     mutable std::atomic<uint64_t> closest_q{0}, any_q{0};
+    // work counters of the bytes model (node visits, tests per kind), when asked
+    bool count = false;
+    mutable std::atomic<uint64_t> nodes{0};
+    mutable std::atomic<uint64_t> tests[6] = {{0}, {0}, {0}, {0}, {0}, {0}};
     Renderer(const Scene& s, const Camera& c, const pcl_settings& se) : S(s), cam(c), st(se) {}
 
-    bool occl(const Ray& r, float a, float b) const { any_q++; return S.occluded(r, a, b, nullptr); }
-    bool hit(const Ray& r, float a, float b, Hit& h) const { closest_q++; return S.hit(r, a, b, h, nullptr, nullptr); }
+    void add(const Counters& c) const {
+        nodes += c.nodes;
+        for (int k = 0; k < 6; ++k) tests[k] += c.tests[k];
+    }
+    bool occl(const Ray& r, float a, float b) const {
+        any_q++;
+        if (!count) return S.occluded(r, a, b, nullptr);
+        Counters c;
+        const bool o = S.occluded(r, a, b, &c);
+        add(c);
+        return o;
+    }
+    bool hit(const Ray& r, float a, float b, Hit& h) const {
+        closest_q++;
+        if (!count) return S.hit(r, a, b, h, nullptr, nullptr);
+        Counters c;
+        const bool x = S.hit(r, a, b, h, nullptr, &c);
+        add(c);
+        return x;
+    }
 
     // Renderer::trace, renderer.hpp:403-822 (branches without textures, fog,
     // SSS, caustics, dispersion and principled: rejected at scene creation).
@@ -944,6 +966,7 @@ int orc_render(void* h, const pcl_camera* c, const pcl_settings* s, float* out, 
     auto& S = ((orc::SceneHandle*)h)->scene;
     orc::Camera cam; cam.init(*c);
     orc::Renderer R(S, cam, *s);
+    R.count = counts != nullptr;
     const int W = s->width;
     int r1 = (row1 <= 0) ? s->height : row1;
     int npx = (r1 - row0) * W;
@@ -957,6 +980,8 @@ int orc_render(void* h, const pcl_camera* c, const pcl_settings* s, float* out, 
         counts->rays_traced += R.rays.load();
         counts->closest_queries += R.closest_q.load();
         counts->anyhit_queries += R.any_q.load();
+        counts->node_visits += R.nodes.load();
+        for (int k = 0; k < 6; ++k) counts->prim_tests[k] += R.tests[k].load();
     }
     return 0;
 }

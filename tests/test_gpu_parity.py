@@ -215,3 +215,40 @@ def test_render_matches_reference_golden_png(core, name):
     assert within >= 0.999
     if TRIG_FREE[name]:
         assert d.max() == 0
+
+
+@pytest.mark.parametrize("scene", ["three_spheres", "shapes", "cornell_box", "soft_shadows"])
+def test_work_counts_match_oracle(core, scene):
+    """The instrumented launch's node visits and primitive tests (the bytes
+    model's units) equal the oracle's instrumented traversal of the same
+    reference-topology BVH."""
+    hs = load(scene, width=96, height=72, samples=2)
+    s = hs.settings()
+    sid = core.upload(hs.desc())
+    d = core.alloc(s.width * s.height * 12)
+    try:
+        st, wc = core.render_counts(sid, hs.camera(), s, d)
+    finally:
+        core.free(d)
+        core.free_scene(sid)
+    _, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    assert st.closest_queries == oc.closest_queries and st.anyhit_queries == oc.anyhit_queries
+    assert wc.node_visits == oc.node_visits
+    assert [wc.prim_tests[k] for k in range(6)] == [oc.prim_tests[k] for k in range(6)]
+
+
+def test_work_counts_match_oracle_stress(core):
+    fx = np.load(golden("bvh_stress.npz"))
+    p = json.loads(str(fx["params"]))
+    hs = pa.HostScene.stress(**p)
+    s = hs.settings()
+    sid = core.upload(hs.desc())
+    d = core.alloc(s.width * s.height * 12)
+    try:
+        st, wc = core.render_counts(sid, hs.camera(), s, d)
+    finally:
+        core.free(d)
+        core.free_scene(sid)
+    _, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    assert wc.node_visits == oc.node_visits
+    assert [wc.prim_tests[k] for k in range(6)] == [oc.prim_tests[k] for k in range(6)]
