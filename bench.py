@@ -222,7 +222,7 @@ def leg_stress(core, pa, args):
 # ---------------------------------------------------------------------------
 
 def leg_cornell_pt(core, pa, args, threads):
-    spp_side = 16
+    spp_side = args.pt_samples
     hs = pa.HostScene.cornell_mesh(CORNELL, width=1920, height=1080, **MESH_IN_BOX)
     desc, cam, st = hs.desc(), hs.camera(), hs.settings()
     st.samples, st.pathtrace, st.pathtrace_bounces = spp_side, 1, 8
@@ -236,8 +236,9 @@ def leg_cornell_pt(core, pa, args, threads):
         plain_ms = (time.perf_counter() - t0) * 1000.0
         q = ps.closest_queries + ps.anyhit_queries
         log(f"cornell_pt plain: {plain_ms:.1f} ms, {q} queries, {q / plain_ms / 1e3:.1f} Mrays/s")
+        cfg = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")]) if args.batched_config else None
         t0 = time.perf_counter()
-        bs = core.render_batch([(sid, cam, st, d_b)])
+        bs = core.render_batch([(sid, cam, st, d_b)], cfg)
         wf_ms = (time.perf_counter() - t0) * 1000.0
         wq = bs.closest_queries + bs.anyhit_queries
         log(f"cornell_pt batched: {wf_ms:.1f} ms, {wq} queries, {wq / wf_ms / 1e3:.1f} Mrays/s")
@@ -452,6 +453,8 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="skip the batched leg of the headline workload")
     ap.add_argument("--legs", default="cornell_pt,batch64,inverse",
                     help="further workloads to run after the headline one (comma list, or 'none')")
+    ap.add_argument("--pt-samples", type=int, default=16,
+                    help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",
                     help="slots,blocks_per_cu,stack_k,iters_per_graph for the batched leg instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
