@@ -15,8 +15,8 @@ reference's own convention (trace() calls only, no shadow rays) is reported
 too.
 
 roofline: the dominant kernel is the render megakernel.  Its algorithmic bytes
-per launch are sum over queries of (32 B ray + 8 B hit) + 64 B per internal
-node entered + the primitive's bytes per test (sphere 16, triangle 48, cube 16,
+per launch are sum over queries of (32 B ray + 8 B hit) + 112 B per 4-wide
+node entered (four child boxes + four refs) + the primitive's bytes per test (sphere 16, triangle 48, cube 16,
 plane 32, cylinder 48, cone 48), counted by an instrumented launch of the
 same frame; divided by the kernel's mean HIP-event duration.
 
@@ -56,6 +56,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PRIM_BYTES = [16, 48, 16, 32, 48, 48]   # sphere, triangle, cube, plane, cylinder, cone
+NODE_BYTES = 112                         # a 4-wide node read: 6 x 16 B of child boxes + 16 B of refs
 HBM_PEAK_GBS = 8000.0
 STRESS = dict(freq=229, n_spheres=4096, seed=1234, width=1920, height=1080)
 CORNELL = os.path.join(ROOT, "pencil_amd", "scenes", "cornell_c5.pecl")
@@ -142,7 +143,7 @@ def leg_stress(core, pa, args):
     cst, wc = core.render_counts(sid, cam, st, d_out)
     queries = cst.closest_queries + cst.anyhit_queries
     tests = [wc.prim_tests[k] for k in range(6)]
-    bytes_per_launch = queries * (32 + 8) + 64 * wc.node_visits + sum(b * t for b, t in zip(PRIM_BYTES, tests))
+    bytes_per_launch = queries * (32 + 8) + NODE_BYTES * wc.node_visits + sum(b * t for b, t in zip(PRIM_BYTES, tests))
     log(f"counts: {cst.closest_queries} closest + {cst.anyhit_queries} any-hit queries, "
         f"{wc.node_visits} node visits, tests {tests}, {bytes_per_launch / 1e9:.2f} GB algorithmic per frame")
 

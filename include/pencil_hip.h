@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 3
+#define PCL_ABI_VERSION 4
 
 enum {
     PCL_OK              = 0,
@@ -159,12 +159,14 @@ typedef struct pcl_hit {
 
 typedef struct pcl_scene_info {
     int32_t n_prims;
-    int32_t n_nodes;        /* internal BVH nodes (64 B each) */
+    int32_t n_nodes;        /* internal nodes of the binary BVH (the reference's topology) */
     int32_t depth;          /* longest root-to-leaf path, in edges */
     int32_t root;           /* >=0 internal node, <0 leaf ~slot, INT32_MIN empty */
     uint64_t device_bytes;  /* nodes + primitives + materials + lights */
     double   build_ms;      /* host BVH build time */
     double   upload_ms;     /* host-to-device upload time */
+    int32_t n_wide_nodes;   /* 4-wide nodes the traversal reads (128 B each) */
+    int32_t stack_need;     /* deepest traversal stack (entries); at most 112 */
 } pcl_scene_info;
 
 typedef struct pcl_ctx pcl_ctx;

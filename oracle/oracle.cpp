@@ -362,7 +362,7 @@ struct Prim {
 // Per-query work counters (node visits = internal nodes whose box passed,
 // prim tests per kind), used for the algorithmic-bytes roofline.
 struct Counters {
-    uint64_t nodes = 0;
+    uint64_t nodes = 0;   // 4-wide nodes entered (heads, see BVH::mark_wide)
     uint64_t tests[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -383,6 +383,39 @@ struct BVH {
         nodes.clear(); splits.clear();
         nodes.reserve(prims.size() * 2);
         root = storage.empty() ? -1 : build_range(0, (int)storage.size());
+        mark_wide();
+    }
+    // Work counts only (not a reference algorithm): the binary nodes that head
+    // a 4-wide node of the device layout.  The root heads one; a head's cut
+    // starts as its two children and grows by splitting the internal member
+    // with the largest box surface area (the first such on ties) until it has
+    // four members or only leaves; every internal member of the cut heads a
+    // node of its own.  A 4-wide node is entered exactly when the recursion
+    // enters its head, so counting entered heads gives the device's count.
+    std::vector<char> wide_head;
+    void mark_wide() {
+        wide_head.assign(nodes.size(), 0);
+        if (root < 0 || nodes[root].leaf >= 0) return;
+        std::vector<int> todo{root};
+        while (!todo.empty()) {
+            const int h = todo.back();
+            todo.pop_back();
+            wide_head[h] = 1;
+            std::vector<int> cut{nodes[h].left, nodes[h].right};
+            while (cut.size() < 4) {
+                int j = -1;
+                float best = -1.0f;
+                for (int k = 0; k < (int)cut.size(); ++k) {
+                    const Node& m = nodes[cut[k]];
+                    if (m.leaf < 0 && surface_area(m.box) > best) { best = surface_area(m.box); j = k; }
+                }
+                if (j < 0) break;
+                const int a = nodes[cut[j]].left, b = nodes[cut[j]].right;
+                cut[j] = a;
+                cut.insert(cut.begin() + j + 1, b);
+            }
+            for (int m : cut) if (nodes[m].leaf < 0) todo.push_back(m);
+        }
     }
     // bvh.hpp:74-104
     int build_range(int start, int end) {
@@ -464,7 +497,7 @@ struct BVH {
         const Node& n = nodes[ni];
         if (!n.box.hit(r, t_min, t_max)) return false;
         if (n.leaf >= 0) { if (c) c->tests[P[n.leaf].kind]++; return P[n.leaf].hit(r, t_min, t_max, out); }
-        if (c) c->nodes++;
+        if (c && wide_head[ni]) c->nodes++;
         bool hl = n.left >= 0 && hit(P, n.left, r, t_min, t_max, out, c);
         bool hr = n.right >= 0 && hit(P, n.right, r, t_min, hl ? out.t : t_max, out, c);
         return hl || hr;
@@ -474,7 +507,7 @@ struct BVH {
         const Node& n = nodes[ni];
         if (!n.box.hit(r, t_min, t_max)) return false;
         if (n.leaf >= 0) { if (c) c->tests[P[n.leaf].kind]++; Hit tmp; return P[n.leaf].hit(r, t_min, t_max, tmp); }
-        if (c) c->nodes++;
+        if (c && wide_head[ni]) c->nodes++;
         if (n.left >= 0 && occluded(P, n.left, r, t_min, t_max, c)) return true;
         if (n.right >= 0 && occluded(P, n.right, r, t_min, t_max, c)) return true;
         return false;
@@ -517,7 +550,7 @@ struct Scene {
             if (prims[n.leaf].hit(r, t_min, t_max, out)) { id = n.leaf; return true; }
             return false;
         }
-        if (c) c->nodes++;
+        if (c && bvh.wide_head[ni]) c->nodes++;
         bool hl = n.left >= 0 && hit_id(n.left, r, t_min, t_max, out, id, c);
         bool hr = n.right >= 0 && hit_id(n.right, r, t_min, hl ? out.t : t_max, out, id, c);
         return hl || hr;
@@ -890,6 +923,13 @@ int orc_split_points(void* h, int32_t* out, int32_t n) {
     if (n < (int)S.bvh.splits.size()) return -1;
     for (size_t i = 0; i < S.bvh.splits.size(); ++i) out[i] = S.bvh.splits[i];
     return (int)S.bvh.splits.size();
+}
+// Number of binary nodes that head a 4-wide node (BVH::mark_wide).
+int orc_wide_heads(void* h) {
+    auto& S = ((orc::SceneHandle*)h)->scene;
+    int n = 0;
+    for (char c : S.bvh.wide_head) n += c ? 1 : 0;
+    return n;
 }
 
 // Scene::hit / Scene::occluded over a ray batch; counts (optional) accumulate

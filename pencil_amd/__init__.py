@@ -89,7 +89,8 @@ class BatchStats(C.Structure):
 
 class SceneInfo(C.Structure):
     _fields_ = [("n_prims", C.c_int32), ("n_nodes", C.c_int32), ("depth", C.c_int32), ("root", C.c_int32),
-                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("upload_ms", C.c_double)]
+                ("device_bytes", C.c_uint64), ("build_ms", C.c_double), ("upload_ms", C.c_double),
+                ("n_wide_nodes", C.c_int32), ("stack_need", C.c_int32)]
 
 
 # pcl_ray / pcl_hit as numpy records (32 B / 8 B)

@@ -290,11 +290,9 @@ static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
         sl.info.device_bytes += bytes;
         return PCL_OK;
     };
-    void *prims = nullptr, *nodes = nullptr, *slot_of = nullptr;
+    void *prims = nullptr, *wide = nullptr, *slot_of = nullptr;
     int rc;
-    if ((rc = keep(sizeof(PrimRec) * n, &prims)) || (rc = keep(sizeof(NodeRec) * (n - 1), &nodes)) ||
-        (rc = keep(sizeof(int32_t) * n, &slot_of)))
-        return rc;
+    if ((rc = keep(sizeof(PrimRec) * n, &prims)) || (rc = keep(sizeof(int32_t) * n, &slot_of))) return rc;
     std::vector<void*> tmp;
     auto scratch = [&](size_t bytes, void** p) -> int {
         HIP_TRY(ctx, hipMalloc(p, bytes < 256 ? 256 : bytes));
@@ -303,9 +301,12 @@ static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
     };
     auto free_tmp = [&]() { for (void* p : tmp) (void)hipFree(p); };
     void *in = nullptr, *boxes = nullptr, *order = nullptr, *ka = nullptr, *kb = nullptr, *pi = nullptr,
-         *pl = nullptr, *nd = nullptr, *sorted = nullptr, *out16 = nullptr, *work = nullptr;
+         *pl = nullptr, *nd = nullptr, *sorted = nullptr, *out16 = nullptr, *work = nullptr, *nodes = nullptr,
+         *flag = nullptr, *widx = nullptr;
     const size_t work_bytes = pcl::lbvh::temp_bytes(n);
-    if ((rc = scratch(sizeof(PrimRec) * n, &in)) || (rc = scratch(sizeof(float) * 6 * n, &boxes)) ||
+    if ((rc = scratch(sizeof(NodeRec) * (n - 1), &nodes)) || (rc = scratch(sizeof(int32_t) * n, &flag)) ||
+        (rc = scratch(sizeof(int32_t) * n, &widx)) ||
+        (rc = scratch(sizeof(PrimRec) * n, &in)) || (rc = scratch(sizeof(float) * 6 * n, &boxes)) ||
         (rc = scratch(sizeof(int32_t) * n, &order)) || (rc = scratch(sizeof(uint64_t) * n, &ka)) ||
         (rc = scratch(sizeof(uint64_t) * n, &kb)) || (rc = scratch(sizeof(int32_t) * n, &pi)) ||
         (rc = scratch(sizeof(int32_t) * n, &pl)) || (rc = scratch(sizeof(int32_t) * n, &nd)) ||
@@ -327,14 +328,27 @@ static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
     if (e == hipSuccess) e = hipMemcpyAsync(h6, out16, sizeof(h6), hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(L.leaf_order.data(), order, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream);
+    int n_wide = 0;
+    if (e == hipSuccess)
+        e = pcl::lbvh::wide_count(ctx->stream, (const int32_t*)nd, n, (int32_t*)flag, (int32_t*)widx, work,
+                                  work_bytes, &n_wide);
+    if (e == hipSuccess && (n_wide <= 0 || n_wide > n - 1)) e = hipErrorUnknown;
+    if (e == hipSuccess && (rc = keep(sizeof(Node4) * n_wide, &wide))) { free_tmp(); return rc; }
+    if (e == hipSuccess)
+        e = pcl::lbvh::wide_emit(ctx->stream, (const NodeRec*)nodes, n, (const int32_t*)nd, (const int32_t*)widx,
+                                 (Node4*)wide);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     free_tmp();
     if (e != hipSuccess) return fail(ctx, PCL_ERR_HIP, std::string("device BVH build: ") + hipGetErrorString(e));
     for (int k = 0; k < 6; ++k) L.root_box[k] = h6[k];
     L.depth = depth;
     L.root = 0;
+    L.wide_root = 0;
+    // even-depth nodes on a path: at most depth / 2 rounded up, each pushing <= 3
+    L.stack_need = 3 * ((depth + 1) / 2);
+    L.n_wide = n_wide;
     D.prims = (const PrimRec*)prims;
-    D.nodes = (const NodeRec*)nodes;
+    D.nodes = (const Node4*)wide;
     D.slot_of = (const int32_t*)slot_of;
     return PCL_OK;
 }
@@ -364,7 +378,7 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
             release(sl);
             return r;
         }
-    } else if ((r = upload(ctx, sl, L.nodes, &D.nodes)) || (r = upload(ctx, sl, L.prims, &D.prims)) ||
+    } else if ((r = upload(ctx, sl, L.wide, &D.nodes)) || (r = upload(ctx, sl, L.prims, &D.prims)) ||
                (r = upload(ctx, sl, L.mats, &D.mats)) || (r = upload(ctx, sl, L.lights, &D.lights)) ||
                (r = upload(ctx, sl, L.slot_of, &D.slot_of))) {
         release(sl);
@@ -372,14 +386,15 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) { release(sl); return fail(ctx, PCL_ERR_HIP, hipGetErrorString(e)); }
-    if (L.depth > kStackK + kStackSpill) {
+    if (L.stack_need > kStackK + kStackSpill) {
         release(sl);
-        return fail(ctx, PCL_ERR_LIMIT, "BVH depth " + std::to_string(L.depth) + " exceeds the traversal stack (" +
+        return fail(ctx, PCL_ERR_LIMIT, "BVH traversal needs " + std::to_string(L.stack_need) +
+                                            " stack entries, more than the traversal stack (" +
                                             std::to_string(kStackK + kStackSpill) + ")");
     }
     D.n_prims = desc->n_prims;
     D.n_lights = desc->n_lights;
-    D.root = L.root;
+    D.root = L.wide_root;
     D.use_bvh = desc->use_bvh ? 1 : 0;   // 1 and 2 both leave a BVH on the device
     for (int k = 0; k < 6; ++k) D.root_box[k] = L.root_box[k];
     for (int k = 0; k < 3; ++k) { D.bg[k] = desc->background[k]; D.amb[k] = desc->ambient[k]; }
@@ -390,6 +405,8 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     sl.info.n_nodes = device_build ? desc->n_prims - 1 : (int32_t)L.nodes.size();
     sl.info.depth = L.depth;
     sl.info.root = L.root;
+    sl.info.n_wide_nodes = device_build ? L.n_wide : (int32_t)L.wide.size();
+    sl.info.stack_need = L.stack_need;
     sl.info.build_ms = t1 - t0;
     sl.info.upload_ms = now_ms() - t1;
     *scene_out = idx;
@@ -433,7 +450,9 @@ int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t
         info->n_nodes = (int32_t)L.nodes.size();
         info->depth = L.depth;
         info->root = L.root;
-        info->device_bytes = L.nodes.size() * sizeof(NodeRec) + L.prims.size() * sizeof(PrimRec) +
+        info->n_wide_nodes = (int32_t)L.wide.size();
+        info->stack_need = L.stack_need;
+        info->device_bytes = L.wide.size() * sizeof(Node4) + L.prims.size() * sizeof(PrimRec) +
                              L.mats.size() * sizeof(MatRec) + L.lights.size() * sizeof(LightRec) +
                              L.slot_of.size() * sizeof(int32_t);
         info->build_ms = now_ms() - t0;

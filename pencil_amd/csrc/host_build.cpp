@@ -183,6 +183,79 @@ struct Builder {
 
 }  // namespace
 
+namespace {
+
+struct CutItem {
+    int32_t ref;
+    const float* box;   // 6 floats in a NodeRec (lb or rb)
+};
+
+float surface_area(const float* b) {
+    const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+struct Collapser {
+    const std::vector<NodeRec>& bin;
+    std::vector<Node4>& out;
+
+    int32_t emit(int32_t r, int32_t& need) {
+        CutItem cut[4];
+        int cnt = 2;
+        cut[0] = {bin[r].left, bin[r].lb};
+        cut[1] = {bin[r].right, bin[r].rb};
+        while (cnt < 4) {
+            int j = -1;
+            float best = -1.0f;
+            for (int k = 0; k < cnt; ++k)
+                if (cut[k].ref >= 0) {
+                    const float a = surface_area(cut[k].box);
+                    if (a > best) { best = a; j = k; }
+                }
+            if (j < 0) break;
+            const NodeRec& m = bin[cut[j].ref];
+            for (int k = cnt; k > j + 1; --k) cut[k] = cut[k - 1];
+            cut[j] = {m.left, m.lb};
+            cut[j + 1] = {m.right, m.rb};
+            ++cnt;
+        }
+        const int32_t idx = (int32_t)out.size();
+        out.push_back(Node4{});
+        int32_t ch[4] = {kEmptyRoot, kEmptyRoot, kEmptyRoot, kEmptyRoot};
+        need = 0;
+        for (int k = 0; k < cnt; ++k) {
+            int32_t sub = 0;
+            ch[k] = cut[k].ref >= 0 ? emit(cut[k].ref, sub) : cut[k].ref;
+            need = std::max(need, (cnt - 1 - k) + sub);
+        }
+        Node4& o = out[idx];
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < cnt;
+            o.lx[k] = used ? cut[k].box[0] : INFINITY;
+            o.ly[k] = used ? cut[k].box[1] : INFINITY;
+            o.lz[k] = used ? cut[k].box[2] : INFINITY;
+            o.hx[k] = used ? cut[k].box[3] : -INFINITY;
+            o.hy[k] = used ? cut[k].box[4] : -INFINITY;
+            o.hz[k] = used ? cut[k].box[5] : -INFINITY;
+            o.child[k] = ch[k];
+            o.pad[k] = 0;
+        }
+        return idx;
+    }
+};
+
+}  // namespace
+
+void collapse_wide(const std::vector<NodeRec>& bin, int32_t root, std::vector<Node4>& out, int32_t& out_root,
+                   int32_t& need) {
+    out.clear();
+    need = 0;
+    if (root < 0) { out_root = root; return; }   // a leaf or an empty scene
+    out.reserve(bin.size() / 2 + 1);
+    Collapser c{bin, out};
+    out_root = c.emit(root, need);
+}
+
 void tonemap_thresholds(float t[256]) {
     auto enc = [](float v) { return (int)(uint8_t)(std::pow(v, 1.0f / 2.2f) * 255.0f + 0.5f); };
     t[0] = -INFINITY;
@@ -316,6 +389,9 @@ int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
         out.slot_of.resize((size_t)n);
         for (int i = 0; i < n; ++i) out.leaf_order[i] = out.slot_of[i] = i;
         out.root = n > 0 ? (n == 1 ? ~0 : 0) : kEmptyRoot;
+        // n >= 2: the device builds the nodes and sets the wide root; n <= 1
+        // needs no node (a single leaf, or nothing)
+        out.wide_root = n >= 2 ? 0 : out.root;
         if (n > 0) { put(out.root_box, boxes[0].mn); put(out.root_box + 3, boxes[0].mx); }
         out.depth = 0;
         return PCL_OK;
@@ -339,6 +415,7 @@ int build_layout(const pcl_scene_desc* d, HostLayout& out, std::string& err) {
         out.root = kEmptyRoot;
     }
     out.depth = B.max_depth;
+    collapse_wide(out.nodes, out.root, out.wide, out.wide_root, out.stack_need);
     out.prims.resize((size_t)n);
     out.leaf_order.resize((size_t)n);
     out.slot_of.resize((size_t)n);

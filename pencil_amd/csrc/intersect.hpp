@@ -295,7 +295,7 @@ PCL_D bool hit_prim(const PrimView& P, const Ray& r, float t_min, float t_max, f
 // BVH traversal
 
 struct DevScene {
-    const NodeRec* nodes;
+    const Node4* nodes;
     const PrimRec* prims;
     const MatRec* mats;
     const LightRec* lights;
@@ -367,7 +367,7 @@ struct LaneStack {
     }
 };
 
-// Work counters of the algorithmic-bytes model: internal nodes entered and
+// Work counters of the algorithmic-bytes model: 4-wide nodes entered and
 // primitive tests per kind.  ON = false compiles them away.
 template <bool ON>
 struct TravCount {
@@ -387,15 +387,24 @@ struct TravCount {
 };
 
 // Scene::hit (ANY = false) / Scene::occluded (ANY = true), src/scene.hpp:55-81,
-// src/bvh.hpp:31-48.  The traversal visits nodes in the recursion's order
-// (left subtree, then right with the narrowed t_max); right children are
-// pushed with their box's entry distance and re-checked against the current
-// t_max when popped, which is the recursion's late box test.  A primitive
-// is accepted when its test passes with t_max = best t so far (t == t_max
-// accepted), so among equal t the last leaf in DFS order wins, as in the
-// reference.  Returns the winning slot (-1 on a miss); for closest hits t_best
-// is its t and t_acc the t_max it was accepted with (needed to rebuild its
-// record: a cone's answer depends on t_max).
+// src/bvh.hpp:31-48, over 4-wide nodes that collapse the reference-topology
+// binary tree.  The recursion tests a node's box when it calls the node, with
+// the t_max of that moment; here all children of a 4-wide node get their slab
+// distances at once, children 1..3 are pushed with their box's entry distance
+// and each is re-checked against the current t_max when popped, so every box
+// of the cut is checked at the moment the recursion would check it.  The
+// binary nodes the cut skips need no test of their own: a skipped node's box
+// is the union of its children's boxes (min/max of the same floats), the slab
+// test's rounding is monotone in the box faces and NaN slabs (0 * inf) impose
+// no bound in either, so a child that passes implies its skipped ancestor
+// passed at that ancestor's (earlier, larger) t_max; and a child that fails
+// visits nothing, as the recursion below a failing ancestor would.  Hence the
+// leaves are met in the recursion's order with the recursion's t_max.  A
+// primitive is accepted when its test passes with t_max = best t so far
+// (t == t_max accepted), so among equal t the last leaf in DFS order wins, as
+// in the reference.  Returns the winning slot (-1 on a miss); for closest hits
+// t_best is its t and t_acc the t_max it was accepted with (needed to rebuild
+// its record: a cone's answer depends on t_max).
 template <bool ANY, int K, int SPILL, class TC>
 PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, LaneStack<K, SPILL>& st,
                    float& t_best, float& t_acc, TC& cnt) {
@@ -428,13 +437,18 @@ PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, La
         if (ref >= 0) {
             cnt.node();
             const float4* np = reinterpret_cast<const float4*>(S.nodes + ref);
-            const float4 a = gload_f4(np), b = gload_f4(np + 1), c = gload_f4(np + 2);
-            const int4 refs = gload_i4(np + 3);
-            float nl, nr;
-            const bool hl = slab_test(a.x, a.y, a.z, a.w, b.x, b.y, s, t_min, t_max, nl);
-            const bool hr = slab_test(b.z, b.w, c.x, c.y, c.z, c.w, s, t_min, t_max, nr);
-            if (hr) st.push(refs.y, nr);
-            if (hl) { ref = refs.x; continue; }
+            const float4 lx = gload_f4(np), ly = gload_f4(np + 1), lz = gload_f4(np + 2);
+            const float4 hx = gload_f4(np + 3), hy = gload_f4(np + 4), hz = gload_f4(np + 5);
+            const int4 ch = gload_i4(np + 6);
+            float n0, n1, n2, n3;
+            const bool p0 = slab_test(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, s, t_min, t_max, n0);
+            const bool p1 = slab_test(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, s, t_min, t_max, n1);
+            const bool p2 = slab_test(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, s, t_min, t_max, n2);
+            const bool p3 = slab_test(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, s, t_min, t_max, n3);
+            if (p3) st.push(ch.w, n3);
+            if (p2) st.push(ch.z, n2);
+            if (p1) st.push(ch.y, n1);
+            if (p0) { ref = ch.x; continue; }
         } else {
             const int slot = ~ref;
             PrimView P = load_prim(S.prims, slot);

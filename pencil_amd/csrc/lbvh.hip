@@ -6,11 +6,14 @@
 // centers' bounds, interleaved into a 30-bit Morton code, made unique by the
 // insertion id in the low 32 bits of a 64-bit key, and radix-sorted; every
 // internal node then finds its key range and split from longest common
-// prefixes, independently; boxes are filled bottom-up, the second child to
-// arrive at a node combining both.  Same NodeRec / PrimRec layout as the host
-// build, so the traversal kernels are unchanged.  The topology is not the
-// reference's SAH tree: hits are the same except where equal t ties (or a
-// grazing box test) resolve by DFS order.
+// prefixes, independently; boxes are filled bottom-up, one node depth per
+// launch.  The binary tree is then collapsed into the 4-wide nodes the
+// traversal reads: every internal node at an even depth heads one, whose
+// children are its own children with each internal child replaced by that
+// child's two children (the host collapse picks cuts by surface area; depth
+// parity needs no top-down pass here).  The topology is not the reference's
+// SAH tree: hits are the same except where equal t ties (or a grazing box
+// test) resolve by DFS order.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -169,14 +172,86 @@ __global__ __launch_bounds__(kBlock) void k_sort_boxes(const uint64_t* k, int n,
     for (int a = 0; a < 6; ++a) sorted[(size_t)s * 6 + a] = p[a];
 }
 
+__global__ __launch_bounds__(kBlock) void k_wide_flags(int n, const int32_t* ndepth, int32_t* flag) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n - 1) flag[i] = (ndepth[i] & 1) == 0 ? 1 : 0;
+}
+
+__device__ __forceinline__ int32_t wide_ref(const int32_t* idx, int32_t ref) { return ref >= 0 ? idx[ref] : ref; }
+
+// One 4-wide node per even-depth internal node i, at idx[i] (exclusive scan
+// of the flags).  Internal grandchildren are two levels deeper, so they head
+// 4-wide nodes of their own.
+__global__ __launch_bounds__(kBlock) void k_wide(int n, const NodeRec* nodes, const int32_t* ndepth,
+                                                 const int32_t* idx, Node4* out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n - 1 || (ndepth[i] & 1) != 0) return;
+    const NodeRec& nd = nodes[i];
+    int32_t ref[4];
+    const float* box[4];
+    int cnt = 0;
+    const int32_t kids[2] = {nd.left, nd.right};
+    const float* kbox[2] = {nd.lb, nd.rb};
+    for (int c = 0; c < 2; ++c) {
+        if (kids[c] >= 0) {
+            const NodeRec& m = nodes[kids[c]];
+            ref[cnt] = wide_ref(idx, m.left); box[cnt++] = m.lb;
+            ref[cnt] = wide_ref(idx, m.right); box[cnt++] = m.rb;
+        } else {
+            ref[cnt] = kids[c]; box[cnt++] = kbox[c];
+        }
+    }
+    Node4 o;
+    for (int k = 0; k < 4; ++k) {
+        const bool used = k < cnt;
+        o.lx[k] = used ? box[k][0] : INFINITY;
+        o.ly[k] = used ? box[k][1] : INFINITY;
+        o.lz[k] = used ? box[k][2] : INFINITY;
+        o.hx[k] = used ? box[k][3] : -INFINITY;
+        o.hy[k] = used ? box[k][4] : -INFINITY;
+        o.hz[k] = used ? box[k][5] : -INFINITY;
+        o.child[k] = used ? ref[k] : kEmptyRoot;
+        o.pad[k] = 0;
+    }
+    out[idx[i]] = o;
+}
+
 }  // namespace
 
 size_t temp_bytes(int n) {
-    size_t sort = 0, red = 0;
+    size_t sort = 0, red = 0, scan = 0;
     const int m = n > 1 ? n : 2;
     hipcub::DeviceRadixSort::SortKeys(nullptr, sort, (const uint64_t*)nullptr, (uint64_t*)nullptr, m);
     hipcub::DeviceReduce::Max(nullptr, red, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
-    return (sort > red ? sort : red) + 256;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
+    size_t b = sort > red ? sort : red;
+    return (b > scan ? b : scan) + 256;
+}
+
+hipError_t wide_count(hipStream_t st, const int32_t* d_ndepth, int n, int32_t* d_flag, int32_t* d_idx,
+                      void* d_temp, size_t temp_size, int* count) {
+    *count = 0;
+    if (n < 2) return hipSuccess;
+    const int gi = (n - 1 + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_wide_flags, dim3(gi), dim3(kBlock), 0, st, n, d_ndepth, d_flag);
+    size_t tb = temp_size;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(d_temp, tb, d_flag, d_idx, n - 1, st);
+    if (e != hipSuccess) return e;
+    int32_t last[2] = {0, 0};
+    e = hipMemcpyAsync(&last[0], d_idx + (n - 2), sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&last[1], d_flag + (n - 2), sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    *count = last[0] + last[1];
+    return hipSuccess;
+}
+
+hipError_t wide_emit(hipStream_t st, const NodeRec* d_nodes, int n, const int32_t* d_ndepth, const int32_t* d_idx,
+                     Node4* d_out) {
+    if (n < 2) return hipSuccess;
+    const int gi = (n - 1 + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_wide, dim3(gi), dim3(kBlock), 0, st, n, d_nodes, d_ndepth, d_idx, d_out);
+    return hipGetLastError();
 }
 
 hipError_t build(hipStream_t st, const PrimRec* d_prims_in, const float* d_boxes, int n, PrimRec* d_prims_out,

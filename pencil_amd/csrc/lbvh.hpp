@@ -26,5 +26,14 @@ hipError_t build(hipStream_t st, const PrimRec* d_prims_in, const float* d_boxes
                  uint64_t* d_keys_a, uint64_t* d_keys_b, int32_t* d_parent_int, int32_t* d_parent_leaf,
                  int32_t* d_ndepth, float* d_sorted_boxes, float* d_out16, int* depth);
 
+// 4-wide nodes of the tree build() left in d_nodes (n >= 2): flags and an
+// exclusive scan over the n - 1 internal nodes (d_flag, d_idx: n - 1 ints
+// each; d_ndepth from build()), *count = number of 4-wide nodes (synchronizes
+// once); wide_emit then writes them to d_out (count x 128 B, root = 0).
+hipError_t wide_count(hipStream_t st, const int32_t* d_ndepth, int n, int32_t* d_flag, int32_t* d_idx,
+                      void* d_temp, size_t temp_size, int* count);
+hipError_t wide_emit(hipStream_t st, const NodeRec* d_nodes, int n, const int32_t* d_ndepth, const int32_t* d_idx,
+                     Node4* d_out);
+
 }  // namespace lbvh
 }  // namespace pcl

@@ -1,8 +1,12 @@
 // scene_layout.hpp - device memory layout of a scene (shared by the host
 // builder and the kernels).  Plain structs, no HIP types.
 //
-//   nodes   : BVH2 internal nodes, 64 B each, both child boxes + both child
-//             refs in one node so one 64-byte read serves a traversal step.
+//   nodes   : the traversal reads 4-wide nodes (Node4, 128 B): each is one
+//             binary internal node of the reference-topology tree together
+//             with up to four of its descendants (a cut of its subtree), the
+//             cut's boxes in SoA and their refs, in the binary tree's
+//             left-first order.  NodeRec (BVH2, 64 B: both child boxes + both
+//             refs) is what the builders produce first.
 //   prims   : one 64-B record per primitive in BVH leaf order (the order of
 //             the reference's BVH::storage_ after its build), so leaf slot =
 //             leaf rank; records carry the insertion id for reporting.
@@ -22,6 +26,17 @@ struct alignas(16) NodeRec {
     int32_t left, right, pad0, pad1;
 };
 static_assert(sizeof(NodeRec) == 64, "node is one 64-byte line");
+
+// Child k of a 4-wide node: box (lx..hz)[k], ref child[k] (>= 0 a Node4
+// index, < 0 a leaf ~slot).  Unused children have the empty box (+inf, -inf),
+// which every slab test rejects, and ref kEmptyRoot.  The traversal reads the
+// first 112 bytes; the last 16 pad the node to one 128-byte line.
+struct alignas(16) Node4 {
+    float lx[4], ly[4], lz[4], hx[4], hy[4], hz[4];
+    int32_t child[4];
+    int32_t pad[4];
+};
+static_assert(sizeof(Node4) == 128, "4-wide node is one 128-byte line");
 
 // Primitive payloads (g[]):
 //   sphere   : center(0-2) radius(3)

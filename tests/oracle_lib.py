@@ -38,6 +38,7 @@ def lib():
             ("orc_scene_destroy", None, [P]),
             ("orc_leaf_order", C.c_int, [P, P, C.c_int32]),
             ("orc_split_points", C.c_int, [P, P, C.c_int32]),
+            ("orc_wide_heads", C.c_int, [P]),
             ("orc_trace_rays", C.c_int, [P, P, C.c_int32, C.c_int32, P, C.POINTER(Counts), C.c_int32]),
             ("orc_hit_records", C.c_int, [P, P, C.c_int32, P, C.c_int32]),
             ("orc_camera_rays", C.c_int, [C.POINTER(pa.Camera), C.c_int32, C.c_int32, P]),
@@ -74,6 +75,10 @@ class OracleScene:
         out = np.empty(max(self.n, 1), dtype=np.int32)
         n = lib().orc_split_points(self.h, _p(out), out.size)
         return out[:n]
+
+    def wide_heads(self):
+        """Binary nodes that head a 4-wide node under the surface-area collapse."""
+        return int(lib().orc_wide_heads(self.h))
 
     def trace_rays(self, rays, any_hit=False, threads=os.cpu_count() or 1, counts=False):
         rays = np.ascontiguousarray(rays, dtype=pa.RAY_DTYPE)

@@ -23,6 +23,9 @@ def test_core_bvh_topology_matches_reference(name):
     assert np.array_equal(order, b["leaves"])
     assert info.n_prims == len(b["leaves"])
     assert info.n_nodes == max(len(b["leaves"]) - 1, 0)
+    # the 4-wide collapse: the oracle's independent marking of cut heads
+    assert info.n_wide_nodes == ol.OracleScene(hs.desc()).wide_heads()
+    assert 0 <= info.stack_need <= 112
 
 
 def test_core_bvh_topology_stress_scenes():
@@ -42,6 +45,9 @@ def test_core_bvh_topology_stress_scenes():
     assert h.hexdigest() == str(fx["full_hash"])
     assert info.n_prims == 1052916 and info.n_nodes == 1052915
     assert 0 < info.depth < 112
+    assert info.n_wide_nodes == osc.wide_heads()
+    # a 4-wide node has 2..4 children, so between a third and all of the binary nodes
+    assert info.n_nodes / 3 <= info.n_wide_nodes <= info.n_nodes and 0 < info.stack_need <= 112
 
 
 def test_parser_counts_and_settings():
@@ -134,7 +140,7 @@ def test_c_abi_exports_every_declared_symbol():
     missing = declared - exported
     assert not missing, missing
     lib = pa.hip_lib()
-    assert lib.pcl_abi_version() == 3
+    assert lib.pcl_abi_version() == 4
     for name in declared:
         assert hasattr(lib, name)
 
