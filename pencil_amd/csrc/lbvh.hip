@@ -221,9 +221,9 @@ __global__ __launch_bounds__(kBlock) void k_wide(int n, const NodeRec* nodes, co
 size_t temp_bytes(int n) {
     size_t sort = 0, red = 0, scan = 0;
     const int m = n > 1 ? n : 2;
-    hipcub::DeviceRadixSort::SortKeys(nullptr, sort, (const uint64_t*)nullptr, (uint64_t*)nullptr, m);
-    hipcub::DeviceReduce::Max(nullptr, red, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
-    hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
+    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, sort, (const uint64_t*)nullptr, (uint64_t*)nullptr, m);
+    (void)hipcub::DeviceReduce::Max(nullptr, red, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan, (const int32_t*)nullptr, (int32_t*)nullptr, m - 1);
     size_t b = sort > red ? sort : red;
     return (b > scan ? b : scan) + 256;
 }

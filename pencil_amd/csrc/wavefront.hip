@@ -236,17 +236,17 @@ PCL_DN void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
 // Slot program results.
 enum : int { ADV_IDLE = 0, ADV_QUERY = 1, ADV_NEED_PIXEL = 2 };
 
-// Job pixels are numbered in 8x8 tiles (row-major tiles, row-major inside a
-// tile) so that a wave's 64 consecutive slots start on one coherent tile, as
-// in the per-pixel megakernel.  Items are padded to whole tiles; a padding
-// pixel returns false.  Sets up render_tile_region's per-pixel state (:313).
+// Job pixels are numbered in 64-pixel row segments (row-major) so that a
+// wave's 64 consecutive slots start on one coherent run of pixels, as in the
+// per-pixel megakernel.  Rows are padded to whole segments; a padding pixel
+// returns false.  Sets up render_tile_region's per-pixel state (:313).
 __device__ bool assign_pixel(const Ctx& C, Lane& L, int64_t p) {
     const int it = find_item(C.items, C.n_items, p);
     const int r = (int)(p - C.items[it].pix_begin);
     const int W = C.items[it].st.width, H = C.items[it].st.height;
-    const int tiles_x = (W + 7) >> 3;
-    const int tile = r >> 6, in = r & 63;
-    const int i = (tile % tiles_x) * 8 + (in & 7), j = (tile / tiles_x) * 8 + (in >> 3);
+    const int segs_x = (W + 63) >> 6;
+    const int seg = r >> 6, in = r & 63;
+    const int i = (seg % segs_x) * 64 + in, j = seg / segs_x;
     if (i >= W || j >= H) return false;
     L.item = it;
     L.pixel = j * W + i;
@@ -327,8 +327,8 @@ __device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint
 }
 
 // Hands new pixels to every lane whose slot finished one: one fetch per wave
-// for the whole wave's requests.  The job's 64-pixel tiles are dealt to the
-// stripes round-robin (tile t to stripe t % kStripes), so while the stripes
+// for the whole wave's requests.  The job's 64-pixel segments ("tiles") are
+// dealt to the stripes round-robin (tile t to stripe t % kStripes), so while the stripes
 // advance together the whole GPU works on one front of the image, whose BVH
 // working set the caches share.  Loops because a new pixel can finish
 // without a query (zero depth / zero bounces) or be a tile's padding.
@@ -359,7 +359,10 @@ __device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t
     }
 }
 
-template <int K>
+// ONE: the job has a single item (one scene, camera and settings), so its
+// records are read with uniform (scalar) loads instead of per lane through
+// the slot's item index.
+template <int K, bool ONE>
 __global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
     __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
@@ -375,7 +378,7 @@ __global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
     while (f.next(C.q.ctr, S_HEAD_CLOSEST, kWave, len, g, base, got)) {
         if ((uint32_t)lane_id() < got) {
             const int s = C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()];
-            const DevScene& D = C.scenes[C.items[S.meta[s].x].scene];
+            const DevScene& D = ONE ? C.scenes[0] : C.scenes[C.items[S.meta[s].x].scene];
             const float4 o = S.ray_o[s], d = S.ray_d[s];
             Ray r;
             r.o = xyz(o);
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
     }
 }
 
-template <int K>
+template <int K, bool ONE>
 __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
     __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
@@ -406,7 +409,7 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
             const size_t k = (size_t)g * C.q.shcap + base + lane_id();
             const int2 who = C.q.sh_who[k];
             const float4 a = C.q.sh_ray[2 * k], b = C.q.sh_ray[2 * k + 1];
-            const DevScene& D = C.scenes[C.items[S.meta[who.x].x].scene];
+            const DevScene& D = ONE ? C.scenes[0] : C.scenes[C.items[S.meta[who.x].x].scene];
             Ray r;
             r.o = f3(a.x, a.y, a.z);
             r.d = f3(b.x, b.y, b.z);
@@ -421,6 +424,7 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // the slot's sample array in the reference's loop order; the shadow rays go
 // straight into the shadow queue at an offset allocated per wave up front
 // (the number of tested samples depends only on the lights and the settings).
+template <bool ONE>
 __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
     const int cur = C.q.ctr[C_CUR];
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
@@ -433,8 +437,8 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
         int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
         float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
         if (s >= 0) { meta = S.meta[s]; ctl = S.ctl[s]; ro = S.ray_o[s]; rd = S.ray_d[s]; }
-        const ItemRec* itp = (s >= 0) ? &C.items[meta.x] : nullptr;
-        const DevScene* Dp = itp ? &C.scenes[itp->scene] : nullptr;
+        const ItemRec* itp = ONE ? &C.items[0] : (s >= 0) ? &C.items[meta.x] : nullptr;
+        const DevScene* Dp = ONE ? &C.scenes[0] : itp ? &C.scenes[itp->scene] : nullptr;
         const bool path = (ctl.x & 15) == M_PATH;
         const int slot = (s >= 0) ? __float_as_int(rd.w) : -1;
         uint32_t n_test = 0;
@@ -608,6 +612,7 @@ __device__ f3 fold_whitted_local(const Slots& S, int s, int ns, const f3& emissi
     return local;
 }
 
+template <bool ONE>
 __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
     const int cur = C.q.ctr[C_CUR];
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
@@ -624,8 +629,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
         if ((uint32_t)lane_id() < got) {
             s = C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()];
             load_lane(S, s, L);
-            const ItemRec& it = C.items[L.item];
-            const DevScene& D = C.scenes[it.scene];
+            const DevScene& D = ONE ? C.scenes[0] : C.scenes[C.items[L.item].scene];
             const f3 amb = ld3(D.amb);
             int status;
             f3 result(0, 0, 0);
@@ -798,13 +802,19 @@ int Engine::ensure(int slots, int items, std::string& err) {
     return PCL_OK;
 }
 
-template <int K>
+template <int K, bool ONE>
 static void launch_iter(const Ctx& c, int grid_trav, int grid_other, hipStream_t st) {
-    hipLaunchKernelGGL(k_closest<K>, dim3(grid_trav), dim3(kWfBlock), 0, st, c);
-    hipLaunchKernelGGL(k_shade, dim3(grid_other), dim3(kWfBlock), 0, st, c);
-    hipLaunchKernelGGL(k_anyhit<K>, dim3(grid_trav), dim3(kWfBlock), 0, st, c);
-    hipLaunchKernelGGL(k_resolve, dim3(grid_other), dim3(kWfBlock), 0, st, c);
+    hipLaunchKernelGGL((k_closest<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
+    hipLaunchKernelGGL(k_shade<ONE>, dim3(grid_other), dim3(kWfBlock), 0, st, c);
+    hipLaunchKernelGGL((k_anyhit<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
+    hipLaunchKernelGGL(k_resolve<ONE>, dim3(grid_other), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_swap, dim3(1), dim3(kStripes), 0, st, c.q);
+}
+
+template <int K>
+static void launch_iter(const Ctx& c, bool one, int grid_trav, int grid_other, hipStream_t st) {
+    if (one) launch_iter<K, true>(c, grid_trav, grid_other, st);
+    else launch_iter<K, false>(c, grid_trav, grid_other, st);
 }
 
 int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
@@ -832,6 +842,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     c.sl = sl;
     c.q = q;
     c.q.total_pixels = total;
+    const bool one = its.size() == 1;   // part of the graph key through graph_items
     const int grid_trav = cus * cfg.blocks_per_cu;
     const int grid_other = cus * 8;   // 4 waves per SIMD, the register limit of shade / resolve
     const bool graph_ok = (graph_exec && graph_key.slots == cfg.slots && graph_key.blocks_per_cu == cfg.blocks_per_cu &&
@@ -842,9 +853,9 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         hipGraph_t g = nullptr;
         WF_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < cfg.iters_per_graph; ++i) {
-            if (cfg.stack_k == 8) launch_iter<8>(c, grid_trav, grid_other, stream);
-            else if (cfg.stack_k == 32) launch_iter<32>(c, grid_trav, grid_other, stream);
-            else launch_iter<16>(c, grid_trav, grid_other, stream);
+            if (cfg.stack_k == 8) launch_iter<8>(c, one, grid_trav, grid_other, stream);
+            else if (cfg.stack_k == 32) launch_iter<32>(c, one, grid_trav, grid_other, stream);
+            else launch_iter<16>(c, one, grid_trav, grid_other, stream);
         }
         WF_TRY(hipStreamEndCapture(stream, &g));
         hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);

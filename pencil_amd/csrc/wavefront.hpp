@@ -13,8 +13,9 @@ namespace wf {
 
 constexpr int kMaxSh = 32;   // light samples per shading point (sum over lights)
 
-// Job pixels per item: the image padded to whole 8x8 tiles (tile order).
-inline int64_t padded_pixels(int w, int h) { return (int64_t)((w + 7) / 8 * 8) * (int64_t)((h + 7) / 8 * 8); }
+// Job pixels per item: the image with each row padded to whole 64-pixel
+// segments (segment order: row-major segments, pixels left to right).
+inline int64_t padded_pixels(int w, int h) { return (int64_t)((w + 63) / 64 * 64) * (int64_t)h; }
 
 // One job item: a scene, its camera frame, settings and device output.
 struct ItemRec {
