@@ -26,8 +26,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // adj != nullptr: the image adjoint is given (VJP).  Otherwise it is the L2
 // loss's, 2 (img - tgt), and the loss itself is reduced into the last slot.
+// 128 VGPRs, 4 waves per SIMD: the 512^2 64-spp gradient step measured
+// 85.7 -> 77.8 ms against the unconstrained 208 VGPRs (2 waves).
 template <int NM>
-__global__ __launch_bounds__(kAdjBlock) void k_adjoint(DevScene S, CamRec C, RSettings st, const float* img,
+__global__ __launch_bounds__(kAdjBlock, 4) void k_adjoint(DevScene S, CamRec C, RSettings st, const float* img,
                                                        const float* tgt, const float* adj, double* partial,
                                                        int n_par) {
     __shared__ unsigned long long lds[kStackK * kAdjBlock];

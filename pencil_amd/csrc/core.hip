@@ -86,10 +86,13 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
     if (COUNT) flush_counts(c, ctr);
 }
 
-// At most 168 VGPRs, 3 waves per SIMD: measured faster than both the
-// unconstrained 183 (2 waves) and 128 (4 waves, more spills).
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock, 3) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
+// One instantiation per tracing mode (PT: Whitted / path), each with the
+// register budget measured fastest for it: Whitted at most 168 VGPRs, 3 waves
+// per SIMD (faster than 183 / 2 waves; 128 / 4 waves is no faster); path
+// tracing 128 VGPRs, 4 waves (the 1M-triangle Cornell frame 1.19x faster
+// than at 168 / 3 waves, and faster than 96 / 5 waves).
+template <bool COUNT, int PT>
+__global__ __launch_bounds__(kBlock, PT == 1 ? 4 : 3) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
                                                    unsigned long long* ctr) {
     __shared__ unsigned long long lds[kStackK * kBlock];
     // A block is 128 consecutive pixels of one row, blocks in row-major
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_render(DevScene S, CamRec C, RSet
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         stk.stride = kBlock;
-        const f3 px = render_pixel(S, C, st, i, j, F, stk, c);
+        const f3 px = render_pixel<PT>(S, C, st, i, j, F, stk, c);
         float* o = out + ((size_t)j * st.width + i) * 3;
         o[0] = px.x; o[1] = px.y; o[2] = px.z;
     }
@@ -559,10 +562,9 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     unsigned long long h[kCtrSlots] = {0};
     HIP_TRY(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(h), ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (counts)
-        hipLaunchKernelGGL(k_render<true>, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
-    else
-        hipLaunchKernelGGL(k_render<false>, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
+    auto kern = st.pathtrace ? (counts ? k_render<true, 1> : k_render<false, 1>)
+                             : (counts ? k_render<true, 0> : k_render<false, 0>);
+    hipLaunchKernelGGL(kern, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));

@@ -381,8 +381,10 @@ PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, F
     return R;
 }
 
-// render_tile_region's per-pixel body (renderer.hpp:313-387, non-adaptive)
-template <class LC>
+// render_tile_region's per-pixel body (renderer.hpp:313-387, non-adaptive).
+// PT: 0 = Whitted trace() only, 1 = trace_path() only (st.pathtrace 1 or 2),
+// -1 = chosen per call from st.pathtrace.
+template <int PT = -1, class LC>
 PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F, Stack& stk,
                       LC& c) {
     const int W = st.width, H = st.height, n = st.samples;
@@ -395,7 +397,7 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
             u += rng.sym() * (0.5f / (W * n));
             v += rng.sym() * (0.5f / (H * n));
             Ray r = camera_ray(C, u, v, rng);
-            if (st.pathtrace) acc += trace_path(S, st, r, rng, F, stk, c);
+            if (PT == 1 || (PT < 0 && st.pathtrace)) acc += trace_path(S, st, r, rng, F, stk, c);
             else acc += whitted(S, st, r, st.max_depth, rng, F, 0, stk, c);
         }
     }

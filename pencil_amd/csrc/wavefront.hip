@@ -424,19 +424,13 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // the slot's sample array in the reference's loop order; the shadow rays go
 // straight into the shadow queue at an offset allocated per wave up front
 // (the number of tested samples depends only on the lights and the settings).
+// One wave's chunk of vertices of stripe g (s = -1: lane without one).  Every
+// lane of the wave calls it: the shadow-queue offsets are allocated per wave.
+// ro/rd: the query ray with the closest-hit result in .w (t_max at
+// acceptance, hit slot).
 template <bool ONE>
-__global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
-    const int cur = C.q.ctr[C_CUR];
-    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
-    const Slots& S = C.sl;
-    Fetcher f(wave_gid() & (kStripes - 1));
-    int g;
-    uint32_t base, got;
-    while (f.next(C.q.ctr, S_HEAD_SHADE, kWave, len, g, base, got)) {
-        const int s = ((uint32_t)lane_id() < got) ? C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()] : -1;
-        int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
-        float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
-        if (s >= 0) { meta = S.meta[s]; ctl = S.ctl[s]; ro = S.ray_o[s]; rd = S.ray_d[s]; }
+__device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 meta, int4 ctl, float4 ro, float4 rd) {
+        const Slots& S = C.sl;
         const ItemRec* itp = ONE ? &C.items[0] : (s >= 0) ? &C.items[meta.x] : nullptr;
         const DevScene* Dp = ONE ? &C.scenes[0] : itp ? &C.scenes[itp->scene] : nullptr;
         const bool path = (ctl.x & 15) == M_PATH;
@@ -449,7 +443,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
             }
         }
         size_t o = (size_t)g * C.q.shcap + wave_append_n(n_test, &C.q.ctr[sidx(S_SHADOW, g)]);
-        if (s < 0) continue;
+        if (s < 0) return;
         const ItemRec& it = *itp;
         const DevScene& D = *Dp;
         const RSettings& st = it.st;
@@ -590,9 +584,59 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
             S.meta[s] = meta;
         }
         S.ctl[s] = make_int4((ctl.x & 15) | flags | (outcome << 8), ctl.y, nf, ns);
+}
+
+template <bool ONE>
+__global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
+    const int cur = C.q.ctr[C_CUR];
+    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
+    const Slots& S = C.sl;
+    Fetcher f(wave_gid() & (kStripes - 1));
+    int g;
+    uint32_t base, got;
+    while (f.next(C.q.ctr, S_HEAD_SHADE, kWave, len, g, base, got)) {
+        const int s = ((uint32_t)lane_id() < got) ? C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()] : -1;
+        int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
+        float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
+        if (s >= 0) { meta = S.meta[s]; ctl = S.ctl[s]; ro = S.ray_o[s]; rd = S.ray_d[s]; }
+        shade_chunk<ONE>(C, g, s, meta, ctl, ro, rd);
     }
 }
 
+// k_closest and k_shade in one pass: the lane shades its vertex right after
+// its traversal, with the ray and the hit still in registers, so the hit
+// never goes through memory and the slot state is read once.
+template <int K, bool ONE>
+__global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
+    __shared__ unsigned long long lds[K * kWfBlock];
+    LaneStack<K, kStackK + kStackSpill - K> st;
+    st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
+    st.stride = kWfBlock;
+    const int cur = C.q.ctr[C_CUR];
+    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
+    const Slots& S = C.sl;
+    TravCount<false> cnt;
+    Fetcher f(wave_gid() & (kStripes - 1));
+    int g;
+    uint32_t base, got;
+    while (f.next(C.q.ctr, S_HEAD_CLOSEST, kWave, len, g, base, got)) {
+        const int s = ((uint32_t)lane_id() < got) ? C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()] : -1;
+        int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
+        float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
+        if (s >= 0) {
+            meta = S.meta[s]; ctl = S.ctl[s]; ro = S.ray_o[s]; rd = S.ray_d[s];
+            const DevScene& D = ONE ? C.scenes[0] : C.scenes[C.items[meta.x].scene];
+            Ray r;
+            r.o = xyz(ro);
+            r.d = xyz(rd);
+            float tb, ta;
+            const int slot = traverse<false>(D, r, 1e-4f, 1e9f, st, tb, ta, cnt);
+            ro.w = ta;
+            rd.w = __int_as_float(slot);
+        }
+        shade_chunk<ONE>(C, g, s, meta, ctl, ro, rd);
+    }
+}
 // Folds one vertex's light samples exactly as the reference's loops do.
 __device__ f3 fold_whitted_local(const Slots& S, int s, int ns, const f3& emission, const f3& albedo, const f3& amb) {
     const float4* SH = S.sh + (size_t)s * kMaxSh;
@@ -804,8 +848,7 @@ int Engine::ensure(int slots, int items, std::string& err) {
 
 template <int K, bool ONE>
 static void launch_iter(const Ctx& c, int grid_trav, int grid_other, hipStream_t st) {
-    hipLaunchKernelGGL((k_closest<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
-    hipLaunchKernelGGL(k_shade<ONE>, dim3(grid_other), dim3(kWfBlock), 0, st, c);
+    hipLaunchKernelGGL((k_closest_shade<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL((k_anyhit<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_resolve<ONE>, dim3(grid_other), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_swap, dim3(1), dim3(kStripes), 0, st, c.q);
