@@ -801,7 +801,12 @@ static double now_ms() {
                                 return e_ == hipErrorOutOfMemory ? PCL_ERR_OOM : PCL_ERR_HIP; } \
     } while (0)
 
-Engine::~Engine() { release(); }
+Engine::~Engine() {
+    release();
+    if (h_pending) (void)hipHostFree(h_pending);
+    for (hipEvent_t& e : ev_pending)
+        if (e) (void)hipEventDestroy(e);
+}
 
 void Engine::release() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -915,18 +920,36 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     hipLaunchKernelGGL(k_reset, dim3(16), dim3(256), 0, stream, c.q);
     hipLaunchKernelGGL(k_start, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c, cfg.slots);
     WF_TRY(hipGetLastError());
-    uint32_t h_pending = 0;
+    if (!h_pending) WF_TRY(hipHostMalloc((void**)&h_pending, 2 * sizeof(uint32_t), hipHostMallocDefault));
+    for (hipEvent_t& e : ev_pending)
+        if (!e) WF_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // Two graph launches in flight: while the host reads the pending count
+    // after launch k, launch k + 1 already runs, so the GPU never idles for
+    // the round trip.  The launch after the last busy one finds every queue
+    // empty and only resets the counters.
     int launches = 0;
+    auto issue = [&]() -> hipError_t {
+        hipError_t e = hipGraphLaunch(graph_exec, stream);
+        const int k = launches++ & 1;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(&h_pending[k], q.ctr + C_PENDING, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipEventRecord(ev_pending[k], stream);
+        return e;
+    };
     const double t_begin = now_ms();
-    while (true) {
-        WF_TRY(hipGraphLaunch(graph_exec, stream));
-        ++launches;
-        WF_TRY(hipMemcpyAsync(&h_pending, q.ctr + C_PENDING, sizeof(h_pending), hipMemcpyDeviceToHost, stream));
-        WF_TRY(hipStreamSynchronize(stream));
-        if (h_pending == 0) break;
+    WF_TRY(issue());
+    WF_TRY(issue());
+    for (int k = 0;; ++k) {
+        WF_TRY(hipEventSynchronize(ev_pending[k & 1]));
+        if (h_pending[k & 1] == 0) break;
+        WF_TRY(issue());   // reuses slot k & 1, already read
         // every iteration retires at least one query of a finite program; a
         // job still running after this long is a bug, reported, not waited on
-        if (now_ms() - t_begin > 300000.0) { err = "wavefront did not drain in 300 s"; return PCL_ERR_LIMIT; }
+        if (now_ms() - t_begin > 300000.0) {
+            (void)hipStreamSynchronize(stream);
+            err = "wavefront did not drain in 300 s";
+            return PCL_ERR_LIMIT;
+        }
     }
     WF_TRY(hipEventRecord(e1, stream));
     std::vector<unsigned long long> h_st(kStatWords);

@@ -96,6 +96,8 @@ struct Engine {
     Config graph_key;
     int graph_items = 0;
     int64_t graph_total = 0;
+    uint32_t* h_pending = nullptr;      // pinned, one word per graph launch in flight
+    hipEvent_t ev_pending[2] = {nullptr, nullptr};
 
     ~Engine();
     void release();
