@@ -67,6 +67,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cfg_dict(c):
+    return {"slots": c.slots, "blocks_per_cu": c.blocks_per_cu, "stack_k": c.stack_k,
+            "iters_per_graph": c.iters_per_graph, "drain_below": c.drain_below}
+
+
+def cfg_str(c):
+    return (f"slots {c.slots}, blocks/CU {c.blocks_per_cu}, stack {c.stack_k}, iters/graph {c.iters_per_graph}, "
+            f"drain below {c.drain_below}")
+
+
 def cpu_threads():
     return min(16, os.cpu_count() or 1)
 
@@ -170,8 +180,7 @@ def leg_stress(core, pa, args):
             tune_ms = 0.0
         else:
             best, tune_ms = core.autotune(job)
-        log(f"autotune: slots {best.slots}, blocks/CU {best.blocks_per_cu}, stack {best.stack_k}, "
-            f"iters/graph {best.iters_per_graph} -> {tune_ms:.3f} ms")
+        log(f"autotune: {cfg_str(best)} -> {tune_ms:.3f} ms")
         for _ in range(args.warmup):
             core.render_batch(job, best)
         wkms = []
@@ -188,8 +197,7 @@ def leg_stress(core, pa, args):
         wms = (t_end - t_start) * 1000.0 / args.steps
         batched = {"ms_per_step": wms, "kernel_ms": sum(wkms) / len(wkms), "queries": wq,
                    "mrays": wq / (wms / 1000.0) / 1e6, "rmse_vs_plain": rmse(a, b), "iterations": bs.iterations,
-                   "config": {"slots": best.slots, "blocks_per_cu": best.blocks_per_cu, "stack_k": best.stack_k,
-                              "iters_per_graph": best.iters_per_graph}}
+                   "config": cfg_dict(bs.used)}
         log(f"batched: {wms:.3f} ms/frame wall, {batched['kernel_ms']:.3f} ms on the stream, "
             f"{batched['mrays']:.1f} Mrays/s, RMSE vs plain {batched['rmse_vs_plain']:.3g}")
         core.free(d_wf)
@@ -237,12 +245,20 @@ def leg_cornell_pt(core, pa, args, threads):
         plain_ms = (time.perf_counter() - t0) * 1000.0
         q = ps.closest_queries + ps.anyhit_queries
         log(f"cornell_pt plain: {plain_ms:.1f} ms, {q} queries, {q / plain_ms / 1e3:.1f} Mrays/s")
-        cfg = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")]) if args.batched_config else None
+        if args.batched_config:
+            cfg = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")])
+        else:
+            # tuned on the same frame at 2x2 spp (a few seconds), rendered at full spp
+            st_t = pa.Settings.from_buffer_copy(st)
+            st_t.samples = 2
+            cfg, tune_ms = core.autotune([(sid, cam, st_t, d_b)])
+            log(f"cornell_pt autotune at 2x2 spp: {cfg_str(cfg)} -> {tune_ms:.1f} ms")
         t0 = time.perf_counter()
         bs = core.render_batch([(sid, cam, st, d_b)], cfg)
         wf_ms = (time.perf_counter() - t0) * 1000.0
         wq = bs.closest_queries + bs.anyhit_queries
-        log(f"cornell_pt batched: {wf_ms:.1f} ms, {wq} queries, {wq / wf_ms / 1e3:.1f} Mrays/s")
+        log(f"cornell_pt batched: {wf_ms:.1f} ms, {wq} queries, {wq / wf_ms / 1e3:.1f} Mrays/s, "
+            f"{bs.iterations} iterations, {bs.finished_pixels} pixels finished after the drain ({cfg_str(bs.used)})")
         a = np.empty((H, W, 3), dtype=np.float32)
         b = np.empty((H, W, 3), dtype=np.float32)
         core.d2h(a, d_a)
@@ -256,7 +272,9 @@ def leg_cornell_pt(core, pa, args, threads):
            "primitives": desc.n_prims, "queries": q, "ref_convention_rays": ps.rays_traced,
            "plain_ms": plain_ms, "plain_mrays_s": q / plain_ms / 1e3,
            "batched_ms": wf_ms, "batched_mrays_s": wq / wf_ms / 1e3, "batched_kernel_ms": bs.kernel_ms,
-           "batched_iterations": bs.iterations, "rmse_batched_vs_plain": rmse(a, b),
+           "batched_iterations": bs.iterations, "batched_config": cfg_dict(bs.used),
+           "batched_pixels_finished_after_drain": bs.finished_pixels,
+           "speedup_batched_vs_plain": plain_ms / wf_ms, "rmse_batched_vs_plain": rmse(a, b),
            "image_mean": float(a.mean())}
     if not args.no_cpu:
         ol = oracle()
@@ -304,9 +322,20 @@ def leg_batch64(core, pa, args, threads):
     outs2 = [core.alloc(st.width * st.height * 12) for _, st in scenes]
     try:
         job = [(sid, hs.camera(), st, d) for sid, (hs, st), d in zip(sids, scenes, outs)]
-        core.render_batch(job)   # warm-up (graph capture, buffers)
+        if args.batched_config:
+            cfg = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")])
+        else:
+            # tuned on the same 64 scenes at 2x2 spp, rendered at full spp
+            tune_job = []
+            for sid, cam, st, d in job:
+                st_t = pa.Settings.from_buffer_copy(st)
+                st_t.samples = 2
+                tune_job.append((sid, cam, st_t, d))
+            cfg, tune_ms = core.autotune(tune_job)
+            log(f"batch64 autotune at 2x2 spp: {cfg_str(cfg)} -> {tune_ms:.1f} ms")
+        core.render_batch(job, cfg)   # warm-up (graph capture, buffers)
         t0 = time.perf_counter()
-        bs = core.render_batch(job)
+        bs = core.render_batch(job, cfg)
         wf_ms = (time.perf_counter() - t0) * 1000.0
         wq = bs.closest_queries + bs.anyhit_queries
         for sid, (hs, st), d in zip(sids, scenes, outs2):   # warm-up
@@ -336,6 +365,7 @@ def leg_batch64(core, pa, args, threads):
            "primitives": n_prims, "pixels": int(sum(st.width * st.height for _, st in scenes)),
            "queries": wq, "batched_ms": wf_ms, "batched_mrays_s": wq / wf_ms / 1e3,
            "batched_kernel_ms": bs.kernel_ms, "batched_iterations": bs.iterations,
+           "batched_config": cfg_dict(bs.used), "batched_pixels_finished_after_drain": bs.finished_pixels,
            "plain_sequential_ms": plain_ms, "plain_sequential_mrays_s": pq / plain_ms / 1e3,
            "max_rmse_batched_vs_plain": err}
     if not args.no_cpu:
@@ -457,7 +487,8 @@ def main():
     ap.add_argument("--pt-samples", type=int, default=16,
                     help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",
-                    help="slots,blocks_per_cu,stack_k,iters_per_graph for the batched leg instead of autotuning")
+                    help="slots,blocks_per_cu,stack_k,iters_per_graph[,drain_below] for the batched legs "
+                         "instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")
     args = ap.parse_args()

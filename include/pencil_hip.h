@@ -251,6 +251,9 @@ typedef struct pcl_wf_config {
     int32_t blocks_per_cu;    /* persistent traversal blocks per CU */
     int32_t stack_k;          /* LDS traversal-stack entries per lane: 8, 16 or 32 */
     int32_t iters_per_graph;  /* wavefront iterations captured per hipGraph replay */
+    int32_t drain_below;      /* once every job pixel is taken and fewer slots than this are
+                                 active, each slot stops at its next sample boundary and one
+                                 per-pixel launch finishes the remaining samples; -1 = never */
 } pcl_wf_config;
 
 typedef struct pcl_batch_stats {
@@ -261,6 +264,7 @@ typedef struct pcl_batch_stats {
     double        kernel_ms;        /* HIP-event time of the whole job on the stream */
     double        total_ms;
     pcl_wf_config used;
+    uint64_t      finished_pixels;  /* pixels whose last samples ran in the drain's finish launch */
 } pcl_batch_stats;
 
 int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const pcl_wf_config* cfg,

@@ -78,13 +78,13 @@ class BatchItem(C.Structure):
 
 class WfConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("blocks_per_cu", C.c_int32), ("stack_k", C.c_int32),
-                ("iters_per_graph", C.c_int32)]
+                ("iters_per_graph", C.c_int32), ("drain_below", C.c_int32)]
 
 
 class BatchStats(C.Structure):
     _fields_ = [("rays_traced", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
                 ("iterations", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("used", WfConfig)]
+                ("used", WfConfig), ("finished_pixels", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):

@@ -643,6 +643,7 @@ static int batch_prepare(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, s
 static pcl_wf_config to_abi(const pcl::wf::Config& c) {
     pcl_wf_config o;
     o.slots = c.slots; o.blocks_per_cu = c.blocks_per_cu; o.stack_k = c.stack_k; o.iters_per_graph = c.iters_per_graph;
+    o.drain_below = c.drain_below;
     return o;
 }
 
@@ -660,6 +661,7 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
         if (cfg->blocks_per_cu > 0) c.blocks_per_cu = cfg->blocks_per_cu;
         if (cfg->stack_k > 0) c.stack_k = cfg->stack_k;
         if (cfg->iters_per_graph > 0) c.iters_per_graph = cfg->iters_per_graph;
+        if (cfg->drain_below != 0) c.drain_below = cfg->drain_below;
     }
     pcl::wf::RunStats rs;
     std::string err;
@@ -673,6 +675,7 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
         stats->kernel_ms = rs.ms;
         stats->total_ms = now_ms() - t0;
         stats->used = to_abi(rs.cfg);
+        stats->finished_pixels = rs.finished;
     }
     return PCL_OK;
 }
@@ -709,12 +712,20 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     for (int64_t v : {(int64_t)1 << 18, (int64_t)1 << 19, (int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22})
         if (v < total) slot_opts.push_back((int)v);
     slot_opts.push_back((int)std::min<int64_t>(total, 1 << 23));
-    for (int knob = 0; knob < 4; ++knob) {
+    // drain thresholds: auto (the traversal grids' lanes) or 4 .. 48 waves'
+    // worth of lanes per CU.  "Never" is not a candidate: the drain only
+    // cuts a tail, which grows with the samples per pixel, so a short tuning
+    // job cannot tell that it pays on the full one.
+    const int lanes_cu = ctx->cus * 64;
+    const std::vector<int> drain_opts{0, 4 * lanes_cu, 12 * lanes_cu, 24 * lanes_cu, 48 * lanes_cu};
+    for (int knob = 0; knob < 5; ++knob) {
         std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 6, 10, 12, 16}
-                              : knob == 2 ? std::vector<int>{8, 16, 32} : std::vector<int>{1, 4, 16};
+                              : knob == 2 ? std::vector<int>{8, 16, 32} : knob == 3 ? std::vector<int>{1, 4, 16}
+                              : drain_opts;
         for (int v : opts) {
             pcl::wf::Config c = cur;
-            (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k : c.iters_per_graph) = v;
+            (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k
+                       : knob == 3 ? c.iters_per_graph : c.drain_below) = v;
             float ms;
             if ((rc = time_cfg(c, ms))) return fail(ctx, rc, "autotune: " + err);
             if (ms < cur_ms) { cur_ms = ms; cur = c; }

@@ -404,4 +404,25 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
     return acc / (float)(n * n);
 }
 
+// render_pixel from sample k0 on: `rng` is the pixel's stream as sample k0
+// finds it and `acc` the sum of samples 0..k0-1, so the result is
+// render_pixel's bit for bit (same sample order, same additions).  Used by
+// the batched tracer to finish pixels it has started.
+template <int PT = -1, class LC>
+PCL_D f3 render_samples(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, int k0, Rng& rng,
+                        f3 acc, Frame* F, Stack& stk, LC& c) {
+    const int W = st.width, H = st.height, n = st.samples;
+    for (int k = k0; k < n * n; ++k) {
+        const int sy = k / n, sx = k - sy * n;
+        float u = (i + (sx + 0.5f) / n) / (float)W;
+        float v = 1.0f - (j + (sy + 0.5f) / n) / (float)H;
+        u += rng.sym() * (0.5f / (W * n));
+        v += rng.sym() * (0.5f / (H * n));
+        Ray r = camera_ray(C, u, v, rng);
+        if (PT == 1 || (PT < 0 && st.pathtrace)) acc += trace_path(S, st, r, rng, F, stk, c);
+        else acc += whitted(S, st, r, st.max_depth, rng, F, 0, stk, c);
+    }
+    return acc / (float)(n * n);
+}
+
 }  // namespace pcl

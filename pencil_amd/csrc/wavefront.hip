@@ -233,8 +233,9 @@ PCL_DN void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
     }
 }
 
-// Slot program results.
-enum : int { ADV_IDLE = 0, ADV_QUERY = 1, ADV_NEED_PIXEL = 2 };
+// Slot program results.  ADV_HANDOFF: the slot stopped at a sample boundary
+// in drain mode (state kept: next sample index, rng, acc) for the finish kernel.
+enum : int { ADV_IDLE = 0, ADV_QUERY = 1, ADV_NEED_PIXEL = 2, ADV_HANDOFF = 4 };
 
 // Job pixels are numbered in 64-pixel row segments (row-major) so that a
 // wave's 64 consecutive slots start on one coherent run of pixels, as in the
@@ -257,13 +258,15 @@ __device__ bool assign_pixel(const Ctx& C, Lane& L, int64_t p) {
 }
 
 // acc += sample color; starts the next sample (ADV_QUERY pending) or writes
-// the pixel (ADV_NEED_PIXEL).
-__device__ int finish_sample(const Ctx& C, Lane& L, const f3& color, uint32_t& rays) {
+// the pixel (ADV_NEED_PIXEL).  In drain mode a pixel with samples left stops
+// before the next sample's first draw (ADV_HANDOFF).
+__device__ int finish_sample(const Ctx& C, Lane& L, const f3& color, uint32_t& rays, bool drain) {
     L.acc += color;
     const ItemRec& it = C.items[L.item];
     const int n = it.st.samples;
     if (L.sample + 1 < n * n) {
         L.sample += 1;
+        if (drain) return ADV_HANDOFF;
         start_sample(C, L, rays);
         return ADV_QUERY;
     }
@@ -276,7 +279,7 @@ __device__ int finish_sample(const Ctx& C, Lane& L, const f3& color, uint32_t& r
 // Folds a Whitted result into the frames (the recursion's return path).
 // ADV_QUERY: a refraction child (ray/depth set) or a new sample is pending;
 // ADV_NEED_PIXEL: the pixel is done.
-__device__ int unwind(const Ctx& C, int s, Lane& L, f3 result, uint32_t& rays) {
+__device__ int unwind(const Ctx& C, int s, Lane& L, f3 result, uint32_t& rays, bool drain) {
     Frame* F = C.sl.frames + (size_t)s * kMaxFrames;
     while (L.nf > 0) {
         Frame& f = F[L.nf - 1];
@@ -304,24 +307,24 @@ __device__ int unwind(const Ctx& C, int s, Lane& L, f3 result, uint32_t& rays) {
         L.R += T * result - T * e;
         result = L.R;
     }
-    return finish_sample(C, L, result, rays);
+    return finish_sample(C, L, result, rays, drain);
 }
 
 // Drives the lane until it needs a closest-hit query (ADV_QUERY, ray set) or
 // a new pixel (ADV_NEED_PIXEL).  Whitted nodes with depth <= 0 return black
 // at once.  status: ADV_QUERY = the lane's ray is a fresh query; 3 = fold
 // `result` into the frames.
-__device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint32_t& rays) {
+__device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint32_t& rays, bool drain) {
     for (int guard = 0; guard < 1 << 20; ++guard) {
-        if (status == ADV_NEED_PIXEL || status == ADV_IDLE) return status;
-        if (status == 3) { status = unwind(C, s, L, result, rays); continue; }
+        if (status == ADV_NEED_PIXEL || status == ADV_IDLE || status == ADV_HANDOFF) return status;
+        if (status == 3) { status = unwind(C, s, L, result, rays, drain); continue; }
         if (L.mode == M_WHITTED) {
             rays++;                                  // trace() entry (:404)
             if (L.depth <= 0) { status = 3; result = f3(0, 0, 0); continue; }
             return ADV_QUERY;
         }
         if (L.depth > 0) return ADV_QUERY;           // path: while (depth > 0)
-        status = finish_sample(C, L, L.R, rays);
+        status = finish_sample(C, L, L.R, rays, drain);
     }
     return ADV_IDLE;
 }
@@ -332,7 +335,7 @@ __device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint
 // advance together the whole GPU works on one front of the image, whose BVH
 // working set the caches share.  Loops because a new pixel can finish
 // without a query (zero depth / zero bounces) or be a tile's padding.
-__device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t& rays) {
+__device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t& rays, bool drain) {
     const uint32_t tiles = (uint32_t)(C.q.total_pixels >> 6);
     auto len = [&](int g) -> uint32_t {
         return (tiles / kStripes + ((uint32_t)g < tiles % kStripes ? 1u : 0u)) * 64u;
@@ -352,7 +355,7 @@ __device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t
                 const uint32_t k = base + r;   // the stripe's k-th pixel
                 if (r < got && assign_pixel(C, L, ((int64_t)(k >> 6) * kStripes + g) * 64 + (k & 63))) {
                     start_sample(C, L, rays);
-                    a = advance(C, s, L, ADV_QUERY, f3(0, 0, 0), rays);
+                    a = advance(C, s, L, ADV_QUERY, f3(0, 0, 0), rays, drain);
                 }
             }
         }
@@ -661,6 +664,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
     const int cur = C.q.ctr[C_CUR];
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
+    const bool drain = C.q.ctr[C_DRAIN] != 0;
     uint32_t rays = 0;
     const int home = wave_gid() & (kStripes - 1);
     Fetcher f(home), pf(home);
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
             }
             case V_P_MISS: {
                 L.R += L.T * ld3(D.bg);
-                status = finish_sample(C, L, L.R, rays);
+                status = finish_sample(C, L, L.R, rays, drain);
                 break;
             }
             default: {   // path vertex with a hit: NEE terms in light order, then ambient
@@ -703,7 +707,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
                     if (!OC[q] && !(__float_as_int(SH[q].w) & (1 << 16))) L.R += f3(SH[q].x, SH[q].y, SH[q].z);
                 L.R += L.T * xyz(S.alb[s]) * amb;
                 if (L.outcome == V_P_END) {
-                    status = finish_sample(C, L, L.R, rays);
+                    status = finish_sample(C, L, L.R, rays, drain);
                 } else if (L.outcome == V_P_GLASS) {
                     L.mode = M_WHITTED;
                     L.glass_root = 1;
@@ -718,9 +722,9 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
             }
             }
             L.outcome = V_NONE;
-            a = advance(C, s, L, status, result, rays);
+            a = advance(C, s, L, status, result, rays, drain);
         }
-        a = refill(C, s, L, a, pf, rays);
+        a = refill(C, s, L, a, pf, rays, drain);
         if (s >= 0) store_lane(S, s, L);
         const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + (cur ^ 1), g)]);
         if (pos >= 0) C.q.ray_q[cur ^ 1][(size_t)g * C.q.qcap + pos] = s;
@@ -742,7 +746,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
     L.mode = 0; L.glass_root = 0; L.outcome_aux = 0; L.outcome = V_NONE; L.depth = 0; L.nf = 0; L.nsh = 0;
     L.o = L.d = L.acc = L.T = L.R = f3(0, 0, 0);
     int a = (s < n_slots) ? ADV_NEED_PIXEL : ADV_IDLE;
-    a = refill(C, s, L, a, pf, rays);
+    a = refill(C, s, L, a, pf, rays, false);
     if (s < n_slots) store_lane(C.sl, s, L);
     const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + cur, g)]);
     if (pos >= 0) C.q.ray_q[cur][(size_t)g * C.q.qcap + pos] = s;
@@ -750,13 +754,18 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
 }
 
 // End of an iteration, one lane per stripe: account the queries, flip the
-// ray queue, clear the stripe's heads, publish the next queue's length.
+// ray queue, clear the stripe's heads, publish the next queue's length and
+// the number of job pixels no slot has taken yet.
 __global__ void k_swap(Queues q) {
     const int g = threadIdx.x;
     const uint32_t cur = q.ctr[C_CUR];
     uint32_t n = q.ctr[sidx(S_COUNT0 + cur, g)];
     uint32_t nsh = q.ctr[sidx(S_SHADOW, g)];
     uint32_t nn = q.ctr[sidx(S_COUNT0 + (cur ^ 1), g)];
+    const uint32_t tiles = (uint32_t)(q.total_pixels >> 6);
+    const uint32_t plen = (tiles / kStripes + ((uint32_t)g < tiles % kStripes ? 1u : 0u)) * 64u;
+    const uint32_t phead = q.ctr[sidx(S_PIXEL, g)];
+    uint32_t pool = phead < plen ? plen - phead : 0u;
     q.ctr[sidx(S_COUNT0 + cur, g)] = 0;
     q.ctr[sidx(S_SHADOW, g)] = 0;
     for (int h = S_HEAD_CLOSEST; h <= S_HEAD_RESOLVE; ++h) q.ctr[sidx(h, g)] = 0;
@@ -765,6 +774,7 @@ __global__ void k_swap(Queues q) {
         n += __shfl_xor(n, m, kWave);
         nsh += __shfl_xor(nsh, m, kWave);
         nn += __shfl_xor(nn, m, kWave);
+        pool += __shfl_xor(pool, m, kWave);
     }
     if (g == 0) {
         q.stats[ST_CLOSEST] += n;
@@ -772,7 +782,56 @@ __global__ void k_swap(Queues q) {
         q.stats[ST_ITERS] += n ? 1 : 0;
         q.ctr[C_CUR] = cur ^ 1;
         q.ctr[C_PENDING] = nn;
+        q.ctr[C_POOL] = pool;
     }
+}
+
+__global__ void k_set_word(uint32_t* p, uint32_t v) { *p = v; }
+
+// After a drained run every slot still holding a pixel stopped at a sample
+// boundary (ADV_HANDOFF); the others hold none (item -1).  Lists the former's
+// records, in slot order within each wave.
+__global__ __launch_bounds__(kWfBlock) void k_finish_list(Ctx C, int n_slots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int4 m = make_int4(-1, 0, 0, 0);
+    if (s < n_slots) m = C.sl.meta[s];
+    const int pos = wave_append(m.x >= 0, &C.q.ctr[C_NFINISH]);
+    if (pos >= 0) {
+        C.q.fin_meta[pos] = m;
+        C.q.fin_acc[pos] = C.sl.acc[s];
+    }
+}
+
+// Finishes the listed pixels [from .. to) one per lane with the per-pixel
+// program (render_samples, the megakernel's loop from the record's next
+// sample on).  The tail of a job -- few pixels with long sample chains, such
+// as pixels that see glass -- then runs without a barrier per query.
+// Register budget as the megakernel's for the same mode.
+template <int PT>
+__global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, const int4* meta, const float4* accs,
+                                                                    int from, int to) {
+    __shared__ unsigned long long lds[kStackK * kWfBlock];
+    const int e = from + blockIdx.x * blockDim.x + threadIdx.x;
+    LaneCount<false> c;
+    if (e < to) {
+        const int4 m = meta[e];
+        const ItemRec& it = C.items[m.x];
+        const DevScene& D = C.scenes[it.scene];
+        alignas(16) unsigned char fbuf[sizeof(Frame) * kMaxFrames];
+        Frame* F = reinterpret_cast<Frame*>(fbuf);
+        Stack stk;
+        stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
+        stk.stride = kWfBlock;
+        const int W = it.st.width;
+        Rng rng((uint32_t)m.w);
+        const f3 px = render_samples<PT>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F, stk, c);
+        float* o = it.out + (size_t)m.y * 3;
+        o[0] = px.x; o[1] = px.y; o[2] = px.z;
+    }
+    unsigned long long* line = &C.q.stats[(1 + (blockIdx.x & (kStripes - 1))) * kStatLine];
+    wave_add_u64(c.rays, line + ST_RAYS);
+    wave_add_u64(c.closest, line + ST_CLOSEST);
+    wave_add_u64(c.anyhit, line + ST_ANY);
 }
 
 // Clears the job counters (a kernel, not a memset, so it may sit in a graph).
@@ -840,6 +899,7 @@ int Engine::ensure(int slots, int items, std::string& err) {
     const size_t qcap = ((S + 63) / 64 + kStripes - 1) / kStripes * 64;   // a stripe's slots
     const size_t Q = qcap * kStripes, SH = Q * kMaxSh;
     A(q.ray_q[0], int32_t, Q) A(q.ray_q[1], int32_t, Q)
+    A(q.fin_meta, int4, S) A(q.fin_acc, float4, S)
     A(q.sh_who, int2, SH) A(q.sh_ray, float4, 2 * SH)
     A(q.ctr, uint32_t, kCtrWords) A(q.stats, unsigned long long, kStatWords)
     q.qcap = (int32_t)qcap;
@@ -875,6 +935,9 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     if (cfg.blocks_per_cu <= 0) cfg.blocks_per_cu = 8;
     if (cfg.stack_k != 8 && cfg.stack_k != 16 && cfg.stack_k != 32) cfg.stack_k = 16;
     if (cfg.iters_per_graph <= 0) cfg.iters_per_graph = 4;
+    // auto: drain once the active slots would no longer fill the persistent
+    // traversal grids (blocks_per_cu blocks of kWfBlock lanes per CU)
+    if (cfg.drain_below == 0) cfg.drain_below = cus * cfg.blocks_per_cu * kWfBlock;
     if (total <= 0) { out = RunStats{}; return PCL_OK; }
     if (total >= (int64_t)1 << 31) { err = "job larger than 2^31 pixels"; return PCL_ERR_LIMIT; }
     int rc = ensure(cfg.slots, (int)items.size(), err);
@@ -913,6 +976,21 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         graph_items = (int)its.size();
         graph_total = total;
     }
+    if (!h_pending)
+        WF_TRY(hipHostMalloc((void**)&h_pending, 2 * kStatusWords * sizeof(uint32_t), hipHostMallocDefault));
+    for (hipEvent_t& e : ev_pending)
+        if (!e) WF_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    bool any_path = false, any_whitted = false;
+    for (const auto& it : its) (it.st.pathtrace ? any_path : any_whitted) = true;
+    // finish kernel over records [from .. to) on stream st
+    auto finish = [&](const int4* m, const float4* a, uint32_t from, uint32_t to, hipStream_t st) -> hipError_t {
+        if (to <= from) return hipSuccess;
+        const dim3 grid((to - from + kWfBlock - 1) / kWfBlock);
+        if (any_path && !any_whitted) hipLaunchKernelGGL(k_finish<1>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
+        else if (!any_path) hipLaunchKernelGGL(k_finish<0>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
+        else hipLaunchKernelGGL(k_finish<-1>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
+        return hipGetLastError();
+    };
     hipEvent_t e0, e1;
     WF_TRY(hipEventCreate(&e0));
     WF_TRY(hipEventCreate(&e1));
@@ -920,10 +998,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     hipLaunchKernelGGL(k_reset, dim3(16), dim3(256), 0, stream, c.q);
     hipLaunchKernelGGL(k_start, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c, cfg.slots);
     WF_TRY(hipGetLastError());
-    if (!h_pending) WF_TRY(hipHostMalloc((void**)&h_pending, 2 * sizeof(uint32_t), hipHostMallocDefault));
-    for (hipEvent_t& e : ev_pending)
-        if (!e) WF_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // Two graph launches in flight: while the host reads the pending count
+    // Two graph launches in flight: while the host reads the status words
     // after launch k, launch k + 1 already runs, so the GPU never idles for
     // the round trip.  The launch after the last busy one finds every queue
     // empty and only resets the counters.
@@ -932,30 +1007,55 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         hipError_t e = hipGraphLaunch(graph_exec, stream);
         const int k = launches++ & 1;
         if (e == hipSuccess)
-            e = hipMemcpyAsync(&h_pending[k], q.ctr + C_PENDING, sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+            e = hipMemcpyAsync(&h_pending[kStatusWords * k], q.ctr, kStatusWords * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipEventRecord(ev_pending[k], stream);
         return e;
     };
     const double t_begin = now_ms();
+    bool drained = false;
     WF_TRY(issue());
     WF_TRY(issue());
     for (int k = 0;; ++k) {
         WF_TRY(hipEventSynchronize(ev_pending[k & 1]));
-        if (h_pending[k & 1] == 0) break;
+        const uint32_t* hp = &h_pending[kStatusWords * (k & 1)];
+        if (hp[C_PENDING] == 0) break;
+        if (!drained && cfg.drain_below > 0 && hp[C_POOL] == 0 && hp[C_PENDING] < (uint32_t)cfg.drain_below) {
+            // takes effect from the launch after the one in flight
+            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, stream, q.ctr + C_DRAIN, 1u);
+            WF_TRY(hipGetLastError());
+            drained = true;
+        }
         WF_TRY(issue());   // reuses slot k & 1, already read
         // every iteration retires at least one query of a finite program; a
         // job still running after this long is a bug, reported, not waited on
         if (now_ms() - t_begin > 300000.0) {
-            (void)hipStreamSynchronize(stream);
+            (void)hipDeviceSynchronize();
             err = "wavefront did not drain in 300 s";
             return PCL_ERR_LIMIT;
         }
+    }
+    uint32_t n_finish = 0;
+    if (drained) {
+        hipLaunchKernelGGL(k_finish_list, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c,
+                           cfg.slots);
+        WF_TRY(hipGetLastError());
+        WF_TRY(hipMemcpyAsync(&h_pending[0], q.ctr + C_NFINISH, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        WF_TRY(hipStreamSynchronize(stream));
+        n_finish = h_pending[0];
+        if (n_finish > (uint32_t)cfg.slots) {
+            (void)hipDeviceSynchronize();
+            err = "finish list longer than the slot pool";
+            return PCL_ERR_LIMIT;
+        }
+        WF_TRY(finish(q.fin_meta, q.fin_acc, 0, n_finish, stream));
     }
     WF_TRY(hipEventRecord(e1, stream));
     std::vector<unsigned long long> h_st(kStatWords);
     WF_TRY(hipMemcpyAsync(h_st.data(), q.stats, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     WF_TRY(hipStreamSynchronize(stream));
-    for (int g = 0; g < kStripes; ++g) h_st[ST_RAYS] += h_st[(1 + g) * kStatLine + ST_RAYS];
+    for (int g = 0; g < kStripes; ++g)
+        for (int w : {ST_RAYS, ST_CLOSEST, ST_ANY}) h_st[w] += h_st[(1 + g) * kStatLine + w];
     float ms = 0.0f;
     WF_TRY(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
@@ -965,6 +1065,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     out.anyhit = h_st[ST_ANY];
     out.iterations = h_st[ST_ITERS];
     out.graph_launches = launches;
+    out.finished = n_finish;
     out.ms = ms;
     out.cfg = cfg;
     return PCL_OK;

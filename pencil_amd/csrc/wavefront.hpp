@@ -47,7 +47,11 @@ struct Slots {
 // atomics on different lines proceed in parallel.
 constexpr int kStripes = 64;   // equal to the wave size: a scan reads one stripe per lane
 constexpr int kLine = 32;      // uint32 words per 128-byte line
-enum : int { C_CUR = 0, C_PENDING = 1 };   // global counters, line 0
+// global counters, line 0: current queue, queries pending after the last
+// iteration, drain flag (set by the host), job pixels not yet assigned after
+// the last iteration, finish-list length
+enum : int { C_CUR = 0, C_PENDING = 1, C_DRAIN = 2, C_POOL = 3, C_NFINISH = 4 };
+constexpr int kStatusWords = 4;   // words 0..3 of line 0, copied to the host after every graph launch
 enum : int {
     S_COUNT0 = 0, S_COUNT1 = 1, S_SHADOW = 2,   // queue lengths per stripe
     S_HEAD_CLOSEST = 3, S_HEAD_SHADE = 4, S_HEAD_ANY = 5, S_HEAD_RESOLVE = 6,   // work heads
@@ -63,6 +67,10 @@ constexpr int kStatWords = (1 + kStripes) * kStatLine;
 
 struct Queues {
     int32_t* ray_q[2];        // slot ids awaiting a closest-hit query, [stripe][qcap]
+    // Pixels the drain's finish kernel completes (at most one per slot), one
+    // record each: (item, pixel, next sample, rng) and (acc, -).
+    int4* fin_meta;
+    float4* fin_acc;
     int2* sh_who;             // shadow queue: (slot, sample index), [stripe][shcap]
     float4* sh_ray;           // shadow queue: (o, t_min), (d, t_max) pairs
     uint32_t* ctr;            // counters, see sidx()
@@ -76,10 +84,16 @@ struct Config {
     int blocks_per_cu = 0;    // persistent traversal blocks per CU; 0 = auto
     int stack_k = 0;          // LDS stack entries per lane (8, 16, 32); 0 = auto
     int iters_per_graph = 0;  // iterations captured per hipGraph; 0 = auto
+    // Drain: once every job pixel is assigned and fewer than this many slots
+    // are active, slots stop at their next sample boundary and one
+    // per-pixel kernel finishes the remaining samples (no iteration
+    // barriers in the tail).  0 = auto, < 0 = never.
+    int drain_below = 0;
 };
 
 struct RunStats {
     unsigned long long rays = 0, closest = 0, anyhit = 0, iterations = 0;
+    unsigned long long finished = 0;   // pixels completed by the drain's finish kernel
     int graph_launches = 0;
     float ms = 0.0f;
     Config cfg;
@@ -96,7 +110,7 @@ struct Engine {
     Config graph_key;
     int graph_items = 0;
     int64_t graph_total = 0;
-    uint32_t* h_pending = nullptr;      // pinned, one word per graph launch in flight
+    uint32_t* h_pending = nullptr;      // pinned, kStatusWords per graph launch in flight
     hipEvent_t ev_pending[2] = {nullptr, nullptr};
 
     ~Engine();

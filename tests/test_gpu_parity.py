@@ -252,3 +252,31 @@ def test_work_counts_match_oracle_stress(core):
     _, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
     assert wc.node_visits == oc.node_visits
     assert [wc.prim_tests[k] for k in range(6)] == [oc.prim_tests[k] for k in range(6)]
+
+
+@pytest.mark.parametrize("depth", [0, 1])
+@pytest.mark.parametrize("name", SCENES)
+def test_depth_zero_and_one_match_wavefront_and_oracle(core, name, depth):
+    """Whitted renders of max_depth 0 and 1 (primary rays only / primary +
+    shadow rays, the shape of the 1M-primitive stress frame): children of
+    metal and glass hits are depth-0 calls that count a trace() and return
+    black.  Megakernel and wavefront bit for bit, and the oracle's images
+    and counts; the golden scenes cover metal, glass (with total internal
+    reflection), soft shadows and depth of field."""
+    hs = load(name, width=96, height=72, samples=2)
+    s = hs.settings()
+    s.max_depth = depth
+    sid = core.upload(hs.desc())
+    try:
+        mk, mst = core.render(sid, hs.camera(), s)
+        wf, wst = core.render_wavefront(sid, hs.camera(), s, pa.WfConfig(drain_below=-1))
+    finally:
+        core.free_scene(sid)
+    assert np.array_equal(mk.view(np.uint32), wf.view(np.uint32))   # bit for bit, signed zeros included
+    assert (mst.rays_traced, mst.closest_queries, mst.anyhit_queries) == \
+        (wst.rays_traced, wst.closest_queries, wst.anyhit_queries)
+    oimg, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    assert (mst.rays_traced, mst.closest_queries, mst.anyhit_queries) == \
+        (oc.rays_traced, oc.closest_queries, oc.anyhit_queries)
+    exact = np.all(mk == oimg, axis=-1).mean()
+    assert exact == 1.0 if TRIG_FREE[name] else exact >= 0.99
