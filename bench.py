@@ -69,12 +69,13 @@ def log(*a):
 
 def cfg_dict(c):
     return {"slots": c.slots, "blocks_per_cu": c.blocks_per_cu, "stack_k": c.stack_k,
-            "iters_per_graph": c.iters_per_graph, "drain_below": c.drain_below}
+            "iters_per_graph": c.iters_per_graph, "drain_below": c.drain_below,
+            "inline_shadows": c.inline_shadows}
 
 
 def cfg_str(c):
     return (f"slots {c.slots}, blocks/CU {c.blocks_per_cu}, stack {c.stack_k}, iters/graph {c.iters_per_graph}, "
-            f"drain below {c.drain_below}")
+            f"drain below {c.drain_below}, inline shadows {c.inline_shadows}")
 
 
 def cpu_threads():
@@ -487,7 +488,7 @@ def main():
     ap.add_argument("--pt-samples", type=int, default=16,
                     help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",
-                    help="slots,blocks_per_cu,stack_k,iters_per_graph[,drain_below] for the batched legs "
+                    help="slots,blocks_per_cu,stack_k,iters_per_graph[,drain_below[,inline_shadows]] for the batched legs "
                          "instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")

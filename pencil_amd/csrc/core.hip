@@ -644,6 +644,7 @@ static pcl_wf_config to_abi(const pcl::wf::Config& c) {
     pcl_wf_config o;
     o.slots = c.slots; o.blocks_per_cu = c.blocks_per_cu; o.stack_k = c.stack_k; o.iters_per_graph = c.iters_per_graph;
     o.drain_below = c.drain_below;
+    o.inline_shadows = c.inline_shadows;
     return o;
 }
 
@@ -662,6 +663,7 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
         if (cfg->stack_k > 0) c.stack_k = cfg->stack_k;
         if (cfg->iters_per_graph > 0) c.iters_per_graph = cfg->iters_per_graph;
         if (cfg->drain_below != 0) c.drain_below = cfg->drain_below;
+        if (cfg->inline_shadows != 0) c.inline_shadows = cfg->inline_shadows;
     }
     pcl::wf::RunStats rs;
     std::string err;
@@ -718,14 +720,16 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     // job cannot tell that it pays on the full one.
     const int lanes_cu = ctx->cus * 64;
     const std::vector<int> drain_opts{0, 4 * lanes_cu, 12 * lanes_cu, 24 * lanes_cu, 48 * lanes_cu};
-    for (int knob = 0; knob < 5; ++knob) {
+    // shadow rays per vertex traced inline: none, one, or up to 4 / 32
+    const std::vector<int> inline_opts{-1, 1, 4, 32};
+    for (int knob = 0; knob < 6; ++knob) {
         std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 6, 10, 12, 16}
                               : knob == 2 ? std::vector<int>{8, 16, 32} : knob == 3 ? std::vector<int>{1, 4, 16}
-                              : drain_opts;
+                              : knob == 4 ? drain_opts : inline_opts;
         for (int v : opts) {
             pcl::wf::Config c = cur;
             (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k
-                       : knob == 3 ? c.iters_per_graph : c.drain_below) = v;
+                       : knob == 3 ? c.iters_per_graph : knob == 4 ? c.drain_below : c.inline_shadows) = v;
             float ms;
             if ((rc = time_cfg(c, ms))) return fail(ctx, rc, "autotune: " + err);
             if (ms < cur_ms) { cur_ms = ms; cur = c; }

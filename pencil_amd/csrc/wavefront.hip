@@ -5,16 +5,17 @@
 // reference's per-pixel program (src/renderer.hpp:309-390: n*n samples in
 // order on the pixel's xorshift stream; Whitted trace() trees or
 // trace_path() chains) as a state machine that stops at every BVH query.
-// One iteration of the machine is five kernels:
+// One iteration of the machine is four kernels:
 //
-//   closest  : persistent; pops the ray queue, Scene::hit -> (slot, t_max at
-//              acceptance) per query                    [LDS ring stack]
-//   shade    : hit record, material, every light sample's contribution and
-//              shadow ray (pushed to the shadow queue), bounce / child ray;
-//              every RNG draw of the vertex happens here, in the reference's
-//              order (light jitter, Russian roulette, bounce)
+//   closest+shade : persistent; pops the ray queue, Scene::hit, then shades
+//              the hit: hit record, material, every light sample's
+//              contribution and shadow ray, bounce / child ray.  Every RNG
+//              draw of the vertex happens here, in the reference's order
+//              (light jitter, Russian roulette, bounce).  A vertex with few
+//              shadow rays traces them here too; the others go to the
+//              shadow queue                             [LDS ring stack]
 //   anyhit   : persistent; pops the shadow queue, Scene::occluded -> one
-//              bit per sample                           [LDS ring stack]
+//              flag per sample                          [LDS ring stack]
 //   resolve  : folds the unoccluded contributions in the reference's order,
 //              unwinds Whitted frames, finishes samples and pixels, pulls the
 //              next pixel of the job, and pushes the slot's next query
@@ -25,8 +26,11 @@
 // so no counter takes more than a small share of the atomics.  Kernel grids
 // are fixed (persistent waves fetch 64-item chunks, home stripe first), so an
 // iteration is captured once as a hipGraph and replayed until the ray queue
-// drains.  Slot count, persistent blocks per CU and the LDS stack depth
-// are the autotuned knobs.
+// drains.  Near the end of a job the slots stop at sample boundaries and one
+// per-pixel launch finishes the remaining samples (the drain, see
+// Engine::run).  Slot count, persistent blocks per CU, the LDS stack depth,
+// iterations per graph, the drain threshold and the inline-shadow limit are
+// the autotuned knobs.
 //
 // Floating-point results are the megakernel's (and so the reference's): every
 // color is accumulated with the same operations in the same order.
@@ -154,6 +158,7 @@ struct Ctx {
     const DevScene* scenes;
     const ItemRec* items;
     int32_t n_items;
+    int32_t inline_max;   // shadow rays per vertex traced in the closest-hit kernel (Config::inline_shadows)
     Slots sl;
     Queues q;
 };
@@ -335,7 +340,14 @@ __device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint
 // advance together the whole GPU works on one front of the image, whose BVH
 // working set the caches share.  Loops because a new pixel can finish
 // without a query (zero depth / zero bounces) or be a tile's padding.
-__device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t& rays, bool drain) {
+__device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t& rays, bool drain, bool pool_left) {
+    if (!pool_left) {   // every job pixel is taken: no fetch
+        if (a == ADV_NEED_PIXEL) {
+            L.item = -1;
+            a = ADV_IDLE;
+        }
+        return a;
+    }
     const uint32_t tiles = (uint32_t)(C.q.total_pixels >> 6);
     auto len = [&](int g) -> uint32_t {
         return (tiles / kStripes + ((uint32_t)g < tiles % kStripes ? 1u : 0u)) * 64u;
@@ -362,41 +374,9 @@ __device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t
     }
 }
 
-// ONE: the job has a single item (one scene, camera and settings), so its
-// records are read with uniform (scalar) loads instead of per lane through
-// the slot's item index.
-template <int K, bool ONE>
-__global__ __launch_bounds__(kWfBlock) void k_closest(Ctx C) {
-    __shared__ unsigned long long lds[K * kWfBlock];
-    LaneStack<K, kStackK + kStackSpill - K> st;
-    st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-    st.stride = kWfBlock;
-    const int cur = C.q.ctr[C_CUR];
-    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
-    const Slots& S = C.sl;
-    TravCount<false> cnt;
-    Fetcher f(wave_gid() & (kStripes - 1));
-    int g;
-    uint32_t base, got;
-    while (f.next(C.q.ctr, S_HEAD_CLOSEST, kWave, len, g, base, got)) {
-        if ((uint32_t)lane_id() < got) {
-            const int s = C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()];
-            const DevScene& D = ONE ? C.scenes[0] : C.scenes[C.items[S.meta[s].x].scene];
-            const float4 o = S.ray_o[s], d = S.ray_d[s];
-            Ray r;
-            r.o = xyz(o);
-            r.d = xyz(d);
-            float tb, ta;
-            const int slot = traverse<false>(D, r, 1e-4f, 1e9f, st, tb, ta, cnt);
-            // the hit rides in the ray's fourth lanes: (o, t_max at acceptance), (d, slot)
-            S.ray_o[s] = make_float4(o.x, o.y, o.z, ta);
-            S.ray_d[s] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
-        }
-    }
-}
-
 template <int K, bool ONE>
 __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
+    if (C.q.ctr[C_PENDING] == 0) return;   // empty iteration
     __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
     st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
@@ -431,8 +411,9 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // lane of the wave calls it: the shadow-queue offsets are allocated per wave.
 // ro/rd: the query ray with the closest-hit result in .w (t_max at
 // acceptance, hit slot).
-template <bool ONE>
-__device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 meta, int4 ctl, float4 ro, float4 rd) {
+template <bool ONE, class ST>
+__device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 meta, int4 ctl, float4 ro, float4 rd,
+                                            ST& stk, uint32_t& n_inline) {
         const Slots& S = C.sl;
         const ItemRec* itp = ONE ? &C.items[0] : (s >= 0) ? &C.items[meta.x] : nullptr;
         const DevScene* Dp = ONE ? &C.scenes[0] : itp ? &C.scenes[itp->scene] : nullptr;
@@ -445,7 +426,10 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                 if (!L.emissive) n_test += path ? 1 : ((L.radius > 0.0f) ? itp->st.soft_samples : 1);
             }
         }
-        size_t o = (size_t)g * C.q.shcap + wave_append_n(n_test, &C.q.ctr[sidx(S_SHADOW, g)]);
+        // a vertex with at most inline_max shadow rays traces them here, right
+        // after its closest hit; the others go through the shadow queue
+        const bool inl = (int)n_test <= C.inline_max;
+        size_t o = (size_t)g * C.q.shcap + wave_append_n(inl ? 0u : n_test, &C.q.ctr[sidx(S_SHADOW, g)]);
         if (s < 0) return;
         const ItemRec& it = *itp;
         const DevScene& D = *Dp;
@@ -505,8 +489,18 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                         }
                     }
                     SH[ns] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8) | skip));
-                    OC[ns] = 0;
-                    if (!L.emissive) {
+                    if (!L.emissive && inl) {
+                        Ray sr;
+                        sr.o = sh_o;
+                        sr.d = Ld;
+                        float tb, ta;
+                        TravCount<false> tc;
+                        OC[ns] = traverse<true>(D, sr, st.shadow_bias, dist - st.shadow_bias, stk, tb, ta, tc) >= 0;
+                        ++n_inline;
+                    } else {
+                        OC[ns] = 0;
+                    }
+                    if (!L.emissive && !inl) {
                         C.q.sh_who[o] = make_int2(s, ns);
                         C.q.sh_ray[2 * o] = make_float4(sh_o.x, sh_o.y, sh_o.z, st.shadow_bias);
                         C.q.sh_ray[2 * o + 1] = make_float4(Ld.x, Ld.y, Ld.z, dist - st.shadow_bias);
@@ -589,28 +583,15 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
         S.ctl[s] = make_int4((ctl.x & 15) | flags | (outcome << 8), ctl.y, nf, ns);
 }
 
-template <bool ONE>
-__global__ __launch_bounds__(kWfBlock, 4) void k_shade(Ctx C) {
-    const int cur = C.q.ctr[C_CUR];
-    auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
-    const Slots& S = C.sl;
-    Fetcher f(wave_gid() & (kStripes - 1));
-    int g;
-    uint32_t base, got;
-    while (f.next(C.q.ctr, S_HEAD_SHADE, kWave, len, g, base, got)) {
-        const int s = ((uint32_t)lane_id() < got) ? C.q.ray_q[cur][(size_t)g * C.q.qcap + base + lane_id()] : -1;
-        int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
-        float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
-        if (s >= 0) { meta = S.meta[s]; ctl = S.ctl[s]; ro = S.ray_o[s]; rd = S.ray_d[s]; }
-        shade_chunk<ONE>(C, g, s, meta, ctl, ro, rd);
-    }
-}
-
-// k_closest and k_shade in one pass: the lane shades its vertex right after
-// its traversal, with the ray and the hit still in registers, so the hit
-// never goes through memory and the slot state is read once.
+// Closest hit and shading in one pass: the lane shades its vertex right
+// after its traversal, with the ray and the hit still in registers, so the
+// hit never goes through memory and the slot state is read once.  ONE: the
+// job has a single item (one scene, camera and settings), so its records
+// are read with uniform (scalar) loads instead of per lane through the
+// slot's item index.
 template <int K, bool ONE>
 __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
+    if (C.q.ctr[C_PENDING] == 0) return;   // empty iteration
     __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
     st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
@@ -619,7 +600,9 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
     TravCount<false> cnt;
-    Fetcher f(wave_gid() & (kStripes - 1));
+    uint32_t n_inline = 0;
+    const int home = wave_gid() & (kStripes - 1);
+    Fetcher f(home);
     int g;
     uint32_t base, got;
     while (f.next(C.q.ctr, S_HEAD_CLOSEST, kWave, len, g, base, got)) {
@@ -637,8 +620,9 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
             ro.w = ta;
             rd.w = __int_as_float(slot);
         }
-        shade_chunk<ONE>(C, g, s, meta, ctl, ro, rd);
+        shade_chunk<ONE>(C, g, s, meta, ctl, ro, rd, st, n_inline);
     }
+    wave_add_u64(n_inline, &C.q.stats[(1 + home) * kStatLine + ST_ANY]);
 }
 // Folds one vertex's light samples exactly as the reference's loops do.
 __device__ f3 fold_whitted_local(const Slots& S, int s, int ns, const f3& emission, const f3& albedo, const f3& amb) {
@@ -664,7 +648,9 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
     const int cur = C.q.ctr[C_CUR];
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
+    if (C.q.ctr[C_PENDING] == 0) return;   // empty iteration
     const bool drain = C.q.ctr[C_DRAIN] != 0;
+    const bool pool_left = C.q.ctr[C_POOL] != 0;   // as of the last swap; it only shrinks
     uint32_t rays = 0;
     const int home = wave_gid() & (kStripes - 1);
     Fetcher f(home), pf(home);
@@ -724,8 +710,15 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
             L.outcome = V_NONE;
             a = advance(C, s, L, status, result, rays, drain);
         }
-        a = refill(C, s, L, a, pf, rays, drain);
-        if (s >= 0) store_lane(S, s, L);
+        a = refill(C, s, L, a, pf, rays, drain, pool_left);
+        if (s >= 0) {
+            if (a == ADV_QUERY) {
+                store_lane(S, s, L);
+            } else {   // idle or handed off: the finish list reads meta and acc only
+                S.meta[s] = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
+                if (a == ADV_HANDOFF) S.acc[s] = f4(L.acc);
+            }
+        }
         const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + (cur ^ 1), g)]);
         if (pos >= 0) C.q.ray_q[cur ^ 1][(size_t)g * C.q.qcap + pos] = s;
     }
@@ -746,7 +739,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
     L.mode = 0; L.glass_root = 0; L.outcome_aux = 0; L.outcome = V_NONE; L.depth = 0; L.nf = 0; L.nsh = 0;
     L.o = L.d = L.acc = L.T = L.R = f3(0, 0, 0);
     int a = (s < n_slots) ? ADV_NEED_PIXEL : ADV_IDLE;
-    a = refill(C, s, L, a, pf, rays, false);
+    a = refill(C, s, L, a, pf, rays, false, true);
     if (s < n_slots) store_lane(C.sl, s, L);
     const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + cur, g)]);
     if (pos >= 0) C.q.ray_q[cur][(size_t)g * C.q.qcap + pos] = s;
@@ -835,8 +828,11 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
 }
 
 // Clears the job counters (a kernel, not a memset, so it may sit in a graph).
+// C_PENDING starts non-zero so the first iteration runs; C_POOL starts at the
+// job's pixel count (the swap kernel keeps both up to date afterwards).
 __global__ void k_reset(Queues q) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kCtrWords; i += gridDim.x * blockDim.x) q.ctr[i] = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kCtrWords; i += gridDim.x * blockDim.x)
+        q.ctr[i] = i == C_PENDING ? 1u : i == C_POOL ? (uint32_t)q.total_pixels : 0u;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kStatWords; i += gridDim.x * blockDim.x) q.stats[i] = 0;
 }
 
@@ -938,6 +934,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     // auto: drain once the active slots would no longer fill the persistent
     // traversal grids (blocks_per_cu blocks of kWfBlock lanes per CU)
     if (cfg.drain_below == 0) cfg.drain_below = cus * cfg.blocks_per_cu * kWfBlock;
+    if (cfg.inline_shadows == 0) cfg.inline_shadows = 1;
     if (total <= 0) { out = RunStats{}; return PCL_OK; }
     if (total >= (int64_t)1 << 31) { err = "job larger than 2^31 pixels"; return PCL_ERR_LIMIT; }
     int rc = ensure(cfg.slots, (int)items.size(), err);
@@ -950,6 +947,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     c.scenes = d_scenes;
     c.items = d_items;
     c.n_items = (int32_t)its.size();
+    c.inline_max = cfg.inline_shadows;
     c.sl = sl;
     c.q = q;
     c.q.total_pixels = total;
@@ -958,6 +956,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     const int grid_other = cus * 8;   // 4 waves per SIMD, the register limit of shade / resolve
     const bool graph_ok = (graph_exec && graph_key.slots == cfg.slots && graph_key.blocks_per_cu == cfg.blocks_per_cu &&
                            graph_key.stack_k == cfg.stack_k && graph_key.iters_per_graph == cfg.iters_per_graph &&
+                           graph_key.inline_shadows == cfg.inline_shadows &&
                            graph_items == (int)its.size() && graph_total == total);
     if (!graph_ok) {
         if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }

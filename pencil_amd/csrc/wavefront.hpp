@@ -89,6 +89,11 @@ struct Config {
     // per-pixel kernel finishes the remaining samples (no iteration
     // barriers in the tail).  0 = auto, < 0 = never.
     int drain_below = 0;
+    // A vertex with at most this many shadow rays (light samples of
+    // non-emissive lights) traces them in the closest-hit kernel right after
+    // shading; vertices with more go through the shadow queue and the any-hit
+    // kernel.  0 = auto (1), < 0 = never.
+    int inline_shadows = 0;
 };
 
 struct RunStats {

@@ -36,7 +36,9 @@ def test_wavefront_equals_megakernel_and_reference(core, fx_name):
                                  dict(slots=4096, blocks_per_cu=16, stack_k=32, iters_per_graph=3),
                                  dict(slots=1000, blocks_per_cu=2, stack_k=16, iters_per_graph=2, drain_below=-1),
                                  dict(slots=1000, blocks_per_cu=2, stack_k=16, iters_per_graph=1,
-                                      drain_below=1 << 30)])
+                                      drain_below=1 << 30),
+                                 dict(slots=2000, blocks_per_cu=4, stack_k=8, iters_per_graph=2, inline_shadows=-1),
+                                 dict(slots=2000, blocks_per_cu=4, stack_k=32, iters_per_graph=2, inline_shadows=32)])
 def test_wavefront_knobs_do_not_change_results(core, cfg):
     hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=64, height=48, samples=2)
     sid = core.upload(hs.desc())
@@ -49,8 +51,9 @@ def test_wavefront_knobs_do_not_change_results(core, cfg):
     assert np.array_equal(wf, mk)
 
 
-@pytest.mark.parametrize("slots,drain", [(1920, -1), (1920, 700), (1920, 1 << 30), (500, 300), (500, 1 << 30)])
-def test_drain_finishes_pixels_bit_for_bit(core, slots, drain):
+@pytest.mark.parametrize("slots,drain,inline", [(1920, -1, -1), (1920, 700, 1), (1920, 1 << 30, -1),
+                                                (500, 300, 1), (500, 1 << 30, 32)])
+def test_drain_finishes_pixels_bit_for_bit(core, slots, drain, inline):
     """Path tracing with Russian roulette (uneven sample chains).  Whatever
     the drain threshold -- also with fewer slots than pixels, where the drain
     waits until every pixel is taken -- the pixels the finish launch
@@ -61,14 +64,15 @@ def test_drain_finishes_pixels_bit_for_bit(core, slots, drain):
     sid = core.upload(hs.desc())
     try:
         mk, mst = core.render(sid, hs.camera(), s)
-        cfg = pa.WfConfig(slots=slots, blocks_per_cu=2, stack_k=16, iters_per_graph=1, drain_below=drain)
+        cfg = pa.WfConfig(slots=slots, blocks_per_cu=2, stack_k=16, iters_per_graph=1, drain_below=drain,
+                          inline_shadows=inline)
         wf, wst = core.render_wavefront(sid, hs.camera(), s, cfg)
     finally:
         core.free_scene(sid)
     assert np.array_equal(wf, mk)
     assert wst.rays_traced == mst.rays_traced
     assert (wst.closest_queries, wst.anyhit_queries) == (mst.closest_queries, mst.anyhit_queries)
-    assert wst.used.drain_below == drain
+    assert (wst.used.drain_below, wst.used.inline_shadows) == (drain, inline)
     if drain < 0:
         assert wst.finished_pixels == 0
     elif drain > slots:
