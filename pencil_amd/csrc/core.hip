@@ -714,22 +714,20 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     for (int64_t v : {(int64_t)1 << 18, (int64_t)1 << 19, (int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22})
         if (v < total) slot_opts.push_back((int)v);
     slot_opts.push_back((int)std::min<int64_t>(total, 1 << 23));
-    // drain thresholds: auto (the traversal grids' lanes) or 4 .. 48 waves'
-    // worth of lanes per CU.  "Never" is not a candidate: the drain only
-    // cuts a tail, which grows with the samples per pixel, so a short tuning
-    // job cannot tell that it pays on the full one.
-    const int lanes_cu = ctx->cus * 64;
-    const std::vector<int> drain_opts{0, 4 * lanes_cu, 12 * lanes_cu, 24 * lanes_cu, 48 * lanes_cu};
-    // shadow rays per vertex traced inline: none, one, or up to 4 / 32
+    // The drain threshold is not tuned: the tail it cuts grows with the
+    // samples per pixel, so a short tuning job cannot rank the candidates (at
+    // 2x2 spp it picked 48 waves' worth of lanes per CU, 3% slower at 256 spp
+    // than the rule, the traversal grids' lanes).  Shadow rays per vertex
+    // traced inline: none, one, or up to 4 / 32.
     const std::vector<int> inline_opts{-1, 1, 4, 32};
-    for (int knob = 0; knob < 6; ++knob) {
+    for (int knob = 0; knob < 5; ++knob) {
         std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 6, 10, 12, 16}
                               : knob == 2 ? std::vector<int>{8, 16, 32} : knob == 3 ? std::vector<int>{1, 4, 16}
-                              : knob == 4 ? drain_opts : inline_opts;
+                              : inline_opts;
         for (int v : opts) {
             pcl::wf::Config c = cur;
             (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k
-                       : knob == 3 ? c.iters_per_graph : knob == 4 ? c.drain_below : c.inline_shadows) = v;
+                       : knob == 3 ? c.iters_per_graph : c.inline_shadows) = v;
             float ms;
             if ((rc = time_cfg(c, ms))) return fail(ctx, rc, "autotune: " + err);
             if (ms < cur_ms) { cur_ms = ms; cur = c; }

@@ -29,8 +29,7 @@
 // drains.  Near the end of a job the slots stop at sample boundaries and one
 // per-pixel launch finishes the remaining samples (the drain, see
 // Engine::run).  Slot count, persistent blocks per CU, the LDS stack depth,
-// iterations per graph, the drain threshold and the inline-shadow limit are
-// the autotuned knobs.
+// iterations per graph and the inline-shadow limit are the autotuned knobs.
 //
 // Floating-point results are the megakernel's (and so the reference's): every
 // color is accumulated with the same operations in the same order.
