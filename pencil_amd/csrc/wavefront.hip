@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
             r.d = f3(b.x, b.y, b.z);
             float tb, ta;
             const bool occ = traverse<true>(D, r, a.w, b.w, st, tb, ta, cnt) >= 0;
-            S.occl[(size_t)who.x * kMaxSh + who.y] = occ ? 1 : 0;
+            S.occl[(size_t)who.y * S.sh_stride + who.x] = occ ? 1 : 0;
         }
     }
 }
@@ -450,8 +450,8 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
             load_mat(D.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
             Rng rng((uint32_t)meta.w);
             S.alb[s] = f4(albedo);
-            float4* SH = S.sh + (size_t)s * kMaxSh;
-            uint8_t* OC = S.occl + (size_t)s * kMaxSh;
+            float4* SH = S.sh + s;   // sample q at SH[q * S.sh_stride]
+            uint8_t* OC = S.occl + s;
             const f3 T = path ? xyz(S.T[s]) : f3(1, 1, 1);
             const f3 sn = rec.normal;
             const f3 sh_o = rec.p + rec.normal * st.shadow_bias;
@@ -487,17 +487,17 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                             c += spec_color * I * (sp * spec);
                         }
                     }
-                    SH[ns] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8) | skip));
+                    SH[ns * S.sh_stride] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8) | skip));
                     if (!L.emissive && inl) {
                         Ray sr;
                         sr.o = sh_o;
                         sr.d = Ld;
                         float tb, ta;
                         TravCount<false> tc;
-                        OC[ns] = traverse<true>(D, sr, st.shadow_bias, dist - st.shadow_bias, stk, tb, ta, tc) >= 0;
+                        OC[ns * S.sh_stride] = traverse<true>(D, sr, st.shadow_bias, dist - st.shadow_bias, stk, tb, ta, tc) >= 0;
                         ++n_inline;
                     } else {
-                        OC[ns] = 0;
+                        OC[ns * S.sh_stride] = 0;
                     }
                     if (!L.emissive && !inl) {
                         C.q.sh_who[o] = make_int2(s, ns);
@@ -625,16 +625,18 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
 }
 // Folds one vertex's light samples exactly as the reference's loops do.
 __device__ f3 fold_whitted_local(const Slots& S, int s, int ns, const f3& emission, const f3& albedo, const f3& amb) {
-    const float4* SH = S.sh + (size_t)s * kMaxSh;
-    const uint8_t* OC = S.occl + (size_t)s * kMaxSh;
+    const float4* SH = S.sh + s;
+    const uint8_t* OC = S.occl + s;
+    const size_t str = (size_t)S.sh_stride;
     f3 local = emission;
     int q = 0;
     while (q < ns) {
-        const int nsamp = (__float_as_int(SH[q].w) >> 8) & 0xFF;
+        const int nsamp = (__float_as_int(SH[q * str].w) >> 8) & 0xFF;
         f3 contrib(0, 0, 0);
         for (int k = 0; k < nsamp; ++k, ++q) {
-            if (OC[q]) continue;
-            contrib += f3(SH[q].x, SH[q].y, SH[q].z);
+            if (OC[q * str]) continue;
+            const float4 v = SH[q * str];
+            contrib += f3(v.x, v.y, v.z);
         }
         local += contrib / (float)nsamp;
     }
@@ -686,10 +688,14 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
                 break;
             }
             default: {   // path vertex with a hit: NEE terms in light order, then ambient
-                const float4* SH = S.sh + (size_t)s * kMaxSh;
-                const uint8_t* OC = S.occl + (size_t)s * kMaxSh;
+                const float4* SH = S.sh + s;
+                const uint8_t* OC = S.occl + s;
+                const size_t str = (size_t)S.sh_stride;
                 for (int q = 0; q < L.nsh; ++q)
-                    if (!OC[q] && !(__float_as_int(SH[q].w) & (1 << 16))) L.R += f3(SH[q].x, SH[q].y, SH[q].z);
+                    if (!OC[q * str]) {
+                        const float4 v = SH[q * str];
+                        if (!(__float_as_int(v.w) & (1 << 16))) L.R += f3(v.x, v.y, v.z);
+                    }
                 L.R += L.T * xyz(S.alb[s]) * amb;
                 if (L.outcome == V_P_END) {
                     status = finish_sample(C, L, L.R, rays, drain);
@@ -890,6 +896,7 @@ int Engine::ensure(int slots, int items, std::string& err) {
     A(sl.acc, float4, S) A(sl.T, float4, S) A(sl.NT, float4, S) A(sl.R, float4, S) A(sl.pend, float4, S)
     A(sl.alb, float4, S) A(sl.gT, float4, S) A(sl.gE, float4, S)
     A(sl.sh, float4, S * kMaxSh) A(sl.occl, uint8_t, S * kMaxSh)
+    sl.sh_stride = (int64_t)S;
     A(sl.frames, Frame, S * kMaxFrames)
     const size_t qcap = ((S + 63) / 64 + kStripes - 1) / kStripes * 64;   // a stripe's slots
     const size_t Q = qcap * kStripes, SH = Q * kMaxSh;

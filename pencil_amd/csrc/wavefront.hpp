@@ -34,8 +34,12 @@ struct Slots {
     int4* ctl;           // mode | glass_root<<4 | glass-discard<<5 | outcome<<8, depth, frames, samples
     float4 *ray_o, *ray_d;   // current ray; after a closest query .w = t_max at acceptance / hit slot
     float4 *acc, *T, *NT, *R, *pend, *alb, *gT, *gE;
-    float4* sh;          // [slot][kMaxSh] light-sample contribution + meta
-    uint8_t* occl;       // [slot][kMaxSh] occluded flags
+    // Light samples of the current vertex, sample-major ([kMaxSh][sh_stride]):
+    // sample q of slot s at q * sh_stride + s, so the lanes of a wave working
+    // on neighbouring slots touch neighbouring records.
+    float4* sh;          // contribution + meta
+    uint8_t* occl;       // occluded flags
+    int64_t sh_stride;   // slots allocated
     Frame* frames;       // [slot][kMaxFrames]
 };
 
