@@ -196,7 +196,8 @@ int pcl_scene_leaf_order(pcl_ctx* ctx, int32_t scene, int32_t* out, int32_t n);
 int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t* leaf_order, int32_t n);
 
 /* Ray-batch traversal: Scene::hit (closest, any_hit = 0) or Scene::occluded
- * (any_hit = 1), src/scene.hpp:55-81.  Host buffers. */
+ * (any_hit = 1), src/scene.hpp:55-81.  Host buffers.  A ray whose interval
+ * [tmin, tmax] is empty or has a NaN bound reports no hit. */
 int pcl_trace_rays(pcl_ctx* ctx, int32_t scene, const pcl_ray* rays, int32_t n,
                    int32_t any_hit, pcl_hit* hits, pcl_stats* stats);
 /* Same, with rays and hits already in device memory. */

@@ -72,7 +72,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
         st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         st.stride = kBlock;
         float tb, ta;
-        const int slot = traverse<ANY>(S, r, b.z, b.w, st, tb, ta, c.trav);
+        // an empty interval misses, as the reference's box test decides; a
+        // NaN bound is taken as empty (the slab test's max/min would drop it)
+        const int slot = (b.z <= b.w) ? traverse<ANY>(S, r, b.z, b.w, st, tb, ta, c.trav) : -1;
         pcl_hit h;
         if (ANY) {
             h.t = slot >= 0 ? 1.0f : -1.0f;
@@ -206,6 +208,7 @@ int check_settings(pcl_ctx* ctx, const pcl_settings* s) {
     if (s->pathtrace && s->pathtrace_bounces > kMaxFrames)
         return fail(ctx, PCL_ERR_LIMIT, "settings: pathtrace_bounces above " + std::to_string(kMaxFrames));
     if (s->soft_shadow_samples < 0) return fail(ctx, PCL_ERR_INVALID, "settings: soft_shadow_samples < 0");
+    if (!std::isfinite(s->shadow_bias)) return fail(ctx, PCL_ERR_INVALID, "settings: shadow_bias must be finite");
     if (s->pathtrace < 0 || s->pathtrace > 2)
         return fail(ctx, PCL_ERR_INVALID, "settings: pathtrace must be 0 (Whitted), 1 (path) or 2 (path + MIS)");
     return PCL_OK;

@@ -324,20 +324,28 @@ PCL_D SlabRay make_slab_ray(const Ray& r) {
 // iff the final t_max >= final t_min.  `near` is the final t_min; a box
 // that passed with some t_max passes with a smaller t_max' iff near <= t_max'
 // (used to re-check deferred right children exactly as the recursion would).
+// The reference's "t0 > t_min ? t0 : t_min" / "t1 < t_max ? t1 : t_max"
+// (src/primitives.hpp:40-46) as IEEE maxNum / minNum (v_max_f32 / v_min_f32,
+// no compare + select).  Same values here: a NaN face (0 * inf, origin on the
+// plane) leaves the bound unchanged in both forms; t_min starts at a positive
+// t_min of the query and only grows, so no signed zero reaches it; t_max can
+// become a zero of either sign, which only meets comparisons with a positive
+// t_min, where the signs compare equal.  near (= t_min) is therefore
+// bit-identical and every decision the same.
 PCL_D bool slab_test(float lx, float ly, float lz, float hx, float hy, float hz, const SlabRay& s,
                      float t_min, float t_max, float& near) {
     float t0 = (lx - s.ox) * s.ix, t1 = (hx - s.ox) * s.ix;
     if (s.nx) { float x = t0; t0 = t1; t1 = x; }
-    t_min = (t0 > t_min) ? t0 : t_min;
-    t_max = (t1 < t_max) ? t1 : t_max;
+    t_min = fmaxf(t0, t_min);
+    t_max = fminf(t1, t_max);
     t0 = (ly - s.oy) * s.iy; t1 = (hy - s.oy) * s.iy;
     if (s.ny) { float x = t0; t0 = t1; t1 = x; }
-    t_min = (t0 > t_min) ? t0 : t_min;
-    t_max = (t1 < t_max) ? t1 : t_max;
+    t_min = fmaxf(t0, t_min);
+    t_max = fminf(t1, t_max);
     t0 = (lz - s.oz) * s.iz; t1 = (hz - s.oz) * s.iz;
     if (s.nz) { float x = t0; t0 = t1; t1 = x; }
-    t_min = (t0 > t_min) ? t0 : t_min;
-    t_max = (t1 < t_max) ? t1 : t_max;
+    t_min = fmaxf(t0, t_min);
+    t_max = fminf(t1, t_max);
     near = t_min;
     return !(t_max < t_min);
 }

@@ -130,6 +130,34 @@ def test_axis_aligned_and_degenerate_rays(core):
         core.free_scene(sid)
 
 
+def test_ray_interval_edges(core):
+    """Intervals starting at 0 or below, empty ones, and a NaN bound (taken
+    as empty): ids and t as the oracle's reference-order traversal, in both
+    closest-hit and any-hit mode."""
+    hs = load("shapes")
+    osc = ol.OracleScene(hs.desc())
+    sid = core.upload(hs.desc())
+    try:
+        rays = ol.camera_rays(hs.camera(), 32, 24)
+        n = len(rays)
+        lo = np.array([0.0, -0.0, -1.0, -np.inf, 5.0, 2.0], dtype=np.float32)
+        hi = np.array([1e9, 1e9, 1e9, 1e9, 1.0, 2.0], dtype=np.float32)
+        rays["tmin"], rays["tmax"] = lo[np.arange(n) % 6], hi[np.arange(n) % 6]
+        for any_hit in (0, 1):
+            g, _ = core.trace_rays(sid, rays, any_hit=bool(any_hit))
+            h = osc.trace_rays(rays, any_hit=bool(any_hit))
+            assert np.array_equal(g["prim_id"], h["prim_id"])
+            if not any_hit:
+                assert np.array_equal(g["t"], h["t"])
+        bad = rays[:8].copy()
+        bad["tmin"][:4] = np.nan
+        bad["tmax"][4:] = np.nan
+        g, _ = core.trace_rays(sid, bad)
+        assert (g["prim_id"] == -1).all()
+    finally:
+        core.free_scene(sid)
+
+
 def test_stress_small_matches_reference(core):
     fx = np.load(golden("bvh_stress.npz"))
     p = json.loads(str(fx["params"]))
