@@ -176,10 +176,14 @@ __device__ __forceinline__ f3 xyz(const float4& v) { return f3(v.x, v.y, v.z); }
 
 // The slot state a kernel works on, held in registers: loaded once, stored
 // once (no read-after-write round trips through global memory).
+// ray_set: o / d were given a new ray here (a new sample, a refraction
+// child); otherwise the slot's ray is the one the shading kernel wrote, which
+// the lane neither loads nor stores.
 struct Lane {
     int item, pixel, sample;
     uint32_t rng;
     int mode, glass_root, outcome_aux, outcome, depth, nf, nsh;
+    bool ray_set;
     f3 o, d, acc, T, R;
 };
 
@@ -192,15 +196,17 @@ __device__ __forceinline__ void load_lane(const Slots& S, int s, Lane& L) {
     L.item = m.x; L.pixel = m.y; L.sample = m.z; L.rng = (uint32_t)m.w;
     L.mode = c.x & 15; L.glass_root = (c.x >> 4) & 1; L.outcome_aux = (c.x >> 5) & 1; L.outcome = c.x >> 8;
     L.depth = c.y; L.nf = c.z; L.nsh = c.w;
-    L.o = xyz(S.ray_o[s]); L.d = xyz(S.ray_d[s]);
+    L.ray_set = false;
     L.acc = xyz(S.acc[s]); L.T = xyz(S.T[s]); L.R = xyz(S.R[s]);
 }
 
 __device__ __forceinline__ void store_lane(const Slots& S, int s, const Lane& L) {
     S.meta[s] = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
     S.ctl[s] = make_int4(pack_flags(L), L.depth, L.nf, L.nsh);
-    S.ray_o[s] = f4(L.o);
-    S.ray_d[s] = f4(L.d);
+    if (L.ray_set) {
+        S.ray_o[s] = f4(L.o);
+        S.ray_d[s] = f4(L.d);
+    }
     S.acc[s] = f4(L.acc);
     S.T[s] = f4(L.T);
     S.R[s] = f4(L.R);
@@ -223,6 +229,7 @@ PCL_DN void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
     L.rng = rng.s;
     L.o = r.o;
     L.d = r.d;
+    L.ray_set = true;
     L.nf = 0;
     L.glass_root = 0;
     if (st.pathtrace) {
@@ -298,6 +305,7 @@ __device__ int unwind(const Ctx& C, int s, Lane& L, f3 result, uint32_t& rays, b
             f.type = FR_GLASS_REFR;
             L.o = f.ro;
             L.d = f.rd;
+            L.ray_set = true;
             L.depth = f.depth - 1;
             return ADV_QUERY;
         } else {
@@ -743,6 +751,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
     L.item = -1; L.pixel = 0; L.sample = 0; L.rng = 0;
     L.mode = 0; L.glass_root = 0; L.outcome_aux = 0; L.outcome = V_NONE; L.depth = 0; L.nf = 0; L.nsh = 0;
     L.o = L.d = L.acc = L.T = L.R = f3(0, 0, 0);
+    L.ray_set = false;
     int a = (s < n_slots) ? ADV_NEED_PIXEL : ADV_IDLE;
     a = refill(C, s, L, a, pf, rays, false, true);
     if (s < n_slots) store_lane(C.sl, s, L);
