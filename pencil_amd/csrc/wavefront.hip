@@ -213,8 +213,9 @@ __device__ __forceinline__ void store_lane(const Slots& S, int s, const Lane& L)
 }
 
 // Camera ray of the lane's current sample (render_tile_region :361-367).
-// Out of line: it is reached from several places of every slot kernel.
-PCL_DN void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
+// Inlined at each call site: as a call it cost the resolve kernel 5.5 % of
+// the path-traced Cornell frame (every wave has lanes starting a sample).
+PCL_D void start_sample(const Ctx& C, Lane& L, uint32_t& rays) {
     const ItemRec& it = C.items[L.item];
     const RSettings& st = it.st;
     const int W = st.width, H = st.height, n = st.samples;
