@@ -208,8 +208,9 @@ __device__ __forceinline__ void store_lane(const Slots& S, int s, const Lane& L)
         S.ray_d[s] = f4(L.d);
     }
     S.acc[s] = f4(L.acc);
-    S.T[s] = f4(L.T);
-    S.R[s] = f4(L.R);
+    // Whitted slots never read T, and R only below a path's glass vertex
+    if (L.mode == M_PATH) S.T[s] = f4(L.T);
+    if (L.mode == M_PATH || L.glass_root) S.R[s] = f4(L.R);
 }
 
 // Camera ray of the lane's current sample (render_tile_region :361-367).
