@@ -37,7 +37,8 @@ Further workloads ("legs", each reported under "legs" in the JSON line):
   inverse      Cornell box at 512x512, 64 spp, 4 bounces, 48 parameters
                (8 materials x albedo + emission): target from the true
                parameters, 200 Adam steps from a seeded U[0.2, 0.8] start;
-               gradient steps per second (forward + adjoint), loss curve ends,
+               gradient steps per second (one pass renders the image and
+               its gradient), loss curve ends,
                and the gradient against central differences at the start.
 """
 import argparse
@@ -434,13 +435,13 @@ def leg_inverse(core, pa, args, threads, steps=200):
         b1, b2, eps = 0.9, 0.999, 1e-8
         mom, vel = np.zeros_like(x), np.zeros_like(x)
         losses = []
-        fwd_adj_ms = []
+        step_ms = []
         core.adjoint_device(sid, cam, st, d_tgt)   # warm-up
         t_start = time.perf_counter()
         for it in range(1, steps + 1):
             g, loss, s = core.adjoint_device(sid, cam, st, d_tgt)
             losses.append(loss)
-            fwd_adj_ms.append(s.kernel_ms)
+            step_ms.append(s.kernel_ms)
             mom = b1 * mom + (1 - b1) * g
             vel = b2 * vel + (1 - b2) * g * g
             x = x - lr * (mom / (1 - b1 ** it)) / (np.sqrt(vel / (1 - b2 ** it)) + eps)
@@ -459,7 +460,7 @@ def leg_inverse(core, pa, args, threads, steps=200):
                        f"{n} materials x (albedo rgb + emission rgb) = {n * 6} parameters, L2 loss, projected Adam "
                        f"(albedo step 0.02 in [0, 1], emission step 0.1, >= 0)",
            "steps": steps, "grad_steps_per_s": steps / total, "ms_per_step": ms_step,
-           "gpu_ms_per_step_forward_plus_adjoint": float(np.mean(fwd_adj_ms)),
+           "gpu_ms_per_step": float(np.mean(step_ms)),
            "loss_first": losses[0], "loss_last": losses[-1], "loss_at_100": losses[min(99, steps - 1)],
            "param_rmse_start_vs_true": None, "fd_check_at_start": fd_err,
            "fd_check_max_rel_err": max(v["rel_err"] for v in fd_err.values())}

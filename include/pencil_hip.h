@@ -297,9 +297,10 @@ int pcl_tonemap_thresholds(float* t256);
  * grad[m*6 + 0..2] = d/d albedo_m, grad[m*6 + 3..5] = d/d emission_m, the
  * order in which a caller perturbs Material::albedo / Material::emission
  * (src/material.hpp:29-60) for finite differences.  The adjoint pass replays
- * every pixel's samples on the pixel's own RNG stream (the same paths as the
- * forward render) and pushes the image adjoint down them; per-pixel partials
- * are accumulated in FP64 and reduced in a fixed order, so the gradient is
+ * every pixel's samples on the pixel's own RNG stream (the same paths and the
+ * same float operations as the forward render, so it yields the rendered
+ * pixel too) and pushes the image adjoint down them; per-pixel partials are
+ * accumulated in FP64 and reduced in a fixed order, so the gradient is
  * bit-reproducible run to run.  Russian roulette's 1/p weight is held
  * constant (its branch is discrete).  At most PCL_ADJ_MAX_MATERIALS
  * materials.  No reference counterpart: the reference has no derivatives; its
@@ -317,9 +318,11 @@ int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* mat
 int pcl_render_vjp_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                           const float* d_adj, double* grad, pcl_stats* stats);
 
-/* L2 image loss  loss = sum_px,c (I - target)^2  of a forward render and its
- * gradient by the adjoint pass.  target_rgb: host W*H*3 floats; grad: host
- * n_materials*6 doubles; loss: host, one double; out_rgb: optional host image. */
+/* L2 image loss  loss = sum_px,c (I - target)^2  of the rendered image I and
+ * its gradient, in one adjoint pass (I is what pcl_render returns, bit for
+ * bit).  target_rgb: host W*H*3 floats; grad: host n_materials*6 doubles;
+ * loss: host, one double; out_rgb: optional host image; stats: the pass's
+ * query counts and time. */
 int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                 const float* target_rgb, double* grad, double* loss, float* out_rgb, pcl_stats* stats);
 /* Same with the target (and the optional image output) in device memory. */

@@ -845,25 +845,34 @@ static int adj_check(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const p
     return PCL_OK;
 }
 
-// Adjoint pass + reduction; copies n_mat*6 (+ the loss) doubles back.
+// Adjoint pass (which renders the image too) + reduction; copies n_mat*6
+// (+ the loss) doubles back and fills stats from the pass's work counters.
 static int adj_run(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const pcl_settings* settings,
-                   const float* img, const float* tgt, const float* adjw, double* grad, double* loss,
-                   float* adj_ms) {
+                   const float* tgt, const float* adjw, float* img_out, double* grad, double* loss, pcl_stats* stats,
+                   double t0) {
     const CamRec C = make_camera(*cam);
     const RSettings st = to_dev(settings);
     int rc = adj_buffers(ctx, st, sl->n_mats);
     if (rc) return rc;
+    unsigned long long h[kCtrSlots] = {0};
+    HIP_TRY(ctx, hipMemsetAsync(ctx->d_ctr, 0, sizeof(h), ctx->stream));
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    HIP_TRY(ctx, pcl::adj::launch(ctx->stream, sl->dev, C, st, sl->n_mats, img, tgt, adjw, ctx->d_adj_partial,
-                                  ctx->d_adj_out));
+    HIP_TRY(ctx, pcl::adj::launch(ctx->stream, sl->dev, C, st, sl->n_mats, tgt, adjw, img_out, ctx->d_adj_partial,
+                                  ctx->d_adj_out, ctx->d_ctr));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    std::vector<double> h((size_t)sl->n_mats * 6 + 1);
-    HIP_TRY(ctx, hipMemcpyAsync(h.data(), ctx->d_adj_out, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+    std::vector<double> hg((size_t)sl->n_mats * 6 + 1);
+    HIP_TRY(ctx, hipMemcpyAsync(hg.data(), ctx->d_adj_out, hg.size() * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    HIP_TRY(ctx, hipEventElapsedTime(adj_ms, ctx->ev0, ctx->ev1));
-    for (int p = 0; p < sl->n_mats * 6; ++p) grad[p] = h[p];
-    if (loss) *loss = h[(size_t)sl->n_mats * 6];
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    for (int p = 0; p < sl->n_mats * 6; ++p) grad[p] = hg[p];
+    if (loss) *loss = hg[(size_t)sl->n_mats * 6];
+    if (stats) {
+        *stats = pcl_stats{};
+        fill_stats(stats, h, ms, t0);
+    }
     return PCL_OK;
 }
 
@@ -875,15 +884,7 @@ int pcl_render_vjp_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, co
     if (!d_adj) return fail(ctx, PCL_ERR_INVALID, "null image adjoint");
     const double t0 = now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    float ms = 0.0f;
-    rc = adj_run(ctx, sl, cam, settings, nullptr, nullptr, d_adj, grad, nullptr, &ms);
-    if (rc) return rc;
-    if (stats) {
-        *stats = pcl_stats{};
-        stats->kernel_ms = ms;
-        stats->total_ms = now_ms() - t0;
-    }
-    return PCL_OK;
+    return adj_run(ctx, sl, cam, settings, nullptr, d_adj, nullptr, grad, nullptr, stats, t0);
 }
 
 int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
@@ -894,23 +895,7 @@ int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     if (!d_target) return fail(ctx, PCL_ERR_INVALID, "null target");
     const double t0 = now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    float* img = d_out_rgb;
-    if (!img) {   // forward image into the context's staging buffer
-        if ((rc = img_buffers(ctx, (size_t)settings->width * settings->height * 3))) return rc;
-        img = ctx->d_img;
-    }
-    pcl_stats fst{};
-    rc = render_impl(ctx, scene, cam, settings, img, &fst, nullptr);
-    if (rc) return rc;
-    float ms = 0.0f;
-    rc = adj_run(ctx, sl, cam, settings, img, d_target, nullptr, grad, loss, &ms);
-    if (rc) return rc;
-    if (stats) {
-        *stats = fst;
-        stats->kernel_ms = fst.kernel_ms + ms;
-        stats->total_ms = now_ms() - t0;
-    }
-    return PCL_OK;
+    return adj_run(ctx, sl, cam, settings, d_target, nullptr, d_out_rgb, grad, loss, stats, t0);
 }
 
 int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
@@ -924,7 +909,7 @@ int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_se
     const size_t n = (size_t)settings->width * settings->height * 3;
     if ((rc = img_buffers(ctx, n))) return rc;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->d_tgt, target_rgb, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
-    rc = pcl_adjoint_device(ctx, scene, cam, settings, ctx->d_tgt, grad, loss, ctx->d_img, stats);
+    rc = pcl_adjoint_device(ctx, scene, cam, settings, ctx->d_tgt, grad, loss, out_rgb ? ctx->d_img : nullptr, stats);
     if (rc) return rc;
     if (out_rgb) HIP_TRY(ctx, hipMemcpy(out_rgb, ctx->d_img, n * sizeof(float), hipMemcpyDeviceToHost));
     if (stats) stats->total_ms = now_ms() - t0;

@@ -120,9 +120,25 @@ def test_grad_is_reproducible(core, case):
     assert np.array_equal(g2, case["grad"]) and l2 == case["loss"]
 
 
+# The adjoint pass renders the image itself: the forward render's bits, from
+# the forward render's queries.
 def test_image_is_the_forward_render(core, case):
-    img, _ = core.render(case["sid"], case["hs"].camera(), case["st"])
+    img, st = core.render(case["sid"], case["hs"].camera(), case["st"])
     assert np.array_equal(img, case["img"])
+    a = case["stats"]
+    assert (a.rays_traced, a.closest_queries, a.anyhit_queries) == (st.rays_traced, st.closest_queries,
+                                                                     st.anyhit_queries)
+
+
+def test_device_entry_without_image_output(core, case):
+    tgt = FIX["target_" + case["mode"]].astype(np.float32)
+    d = core.alloc(tgt.nbytes)
+    try:
+        core.h2d(d, tgt)
+        g, l, _ = core.adjoint_device(case["sid"], case["hs"].camera(), case["st"], d)
+    finally:
+        core.free(d)
+    assert np.array_equal(g, case["grad"]) and l == case["loss"]
 
 
 def test_vjp_equals_adjoint(core, case):
