@@ -2,8 +2,9 @@
 // to every material's albedo and emission, by replaying the pixel's paths.
 //
 // The adjoint pass re-runs render_pixel's program (the same RNG draws, the
-// same float operations, so the same paths and the same values) and pushes
-// the pixel's adjoint weight W = dLoss/dI / n^2 down each sample:
+// same float operations, so the same paths and the same values) and pushes a
+// weight W down each sample (k_adjoint pushes 1 and scales the pixel's sums
+// by its image adjoint afterwards; every term is channel-diagonal):
 //
 //   Whitted trees (trace(), renderer.hpp:403-822): every node carries the
 //   weight of its value in the root.  A frame stores it when pushed; leaf
@@ -19,7 +20,9 @@
 //   over the tape forms the suffix radiance S_k, giving
 //   dR/da_k = T_k (Q_k + s_k S_{k+1}) and dR/de_k = T_k without dividing by
 //   any albedo.  A glass vertex hands W*T to the Whitted adjoint of its
-//   subtree.
+//   subtree.  trace_path_mis (renderer.hpp:1205-1335) differs in the light
+//   term only, albedo * I * (ndotl w_light + w_brdf) with geometric weights:
+//   its Q_k part is that term at unit albedo.
 //
 // Products are formed in float like the forward pass and accumulated in FP64
 // per lane.
@@ -137,10 +140,12 @@ PCL_D f3 whitted_adj(const DevScene& S, const RSettings& st, Ray r, int depth, R
     }
 }
 
-// trace_path() of render.hpp with the adjoint for a sample of weight W.
+// trace_path() of render.hpp with the adjoint for a sample of weight W
+// (st.pathtrace == 2: the MIS light loop of trace_path_mis).
 template <class LC, int NM>
 PCL_D f3 trace_path_adj(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, AdjFrame* F, Stack& stk, LC& c,
                         const f3& W, Grad<NM>& G) {
+    const bool mis = st.pathtrace == 2;
     c.rays++;
     const int bounces = st.bounces;
     // tape: material, throughput, d/d albedo of the vertex's own terms, and
@@ -180,8 +185,15 @@ PCL_D f3 trace_path_adj(const DevScene& S, const RSettings& st, Ray cur, Rng& rn
             sr.o = rec.p + rec.normal * st.shadow_bias; sr.d = Ld;
             if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
             float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
-            R += T * albedo * ld3(L.I) * ndotl;
-            q += ld3(L.I) * ndotl;
+            if (!mis) {
+                R += T * albedo * ld3(L.I) * ndotl;
+                q += ld3(L.I) * ndotl;
+            } else if (!(ndotl <= 0.0f)) {
+                // the MIS term is albedo * (I ndotl w_light + I w_brdf) with weights
+                // set by the geometry alone: its albedo derivative is the term at unit albedo
+                R += T * mis_light_term(L, albedo, ld3(L.I), Ld, dist, ndotl);
+                q += mis_light_term(L, f3(1, 1, 1), ld3(L.I), Ld, dist, ndotl);
+            }
         }
         R += T * albedo * ld3(S.amb);
         q += ld3(S.amb);

@@ -837,8 +837,6 @@ static int adj_check(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const p
     if (!cam || !grad) return fail(ctx, PCL_ERR_INVALID, "null camera or gradient output");
     int rc = check_settings(ctx, settings);
     if (rc) return rc;
-    if (settings->pathtrace == 2)
-        return fail(ctx, PCL_ERR_UNSUPPORTED, "adjoint: the MIS path tracer has no adjoint here (pathtrace 0 or 1)");
     if (sl->n_mats > PCL_ADJ_MAX_MATERIALS)
         return fail(ctx, PCL_ERR_LIMIT, "adjoint: " + std::to_string(sl->n_mats) + " materials, at most " +
                                             std::to_string(PCL_ADJ_MAX_MATERIALS));

@@ -6,8 +6,9 @@ seeded initial point, every material's albedo and emission drawn from
 U[0.2, 0.8], written as tests/golden/adj_init.pecl (colours are parsed as
 text / 100, so the .pecl carries the exact values both renderers see).
 
-For each mode (path tracing with 4 bounces -- Russian roulette never fires --
-and Whitted with depth 4) the reference renderer, through
+For each mode (path tracing with 4 bounces -- Russian roulette never fires --,
+the same with MIS light sampling, and Whitted with depth 4) the reference
+renderer, through
 oracle/_ref/ref_harness, gives:
   target_<mode>   HDR target image (true parameters)
   loss_<mode>     sum (I - target)^2 at the initial point
@@ -36,6 +37,7 @@ W = H = 24
 MODES = {
     "pt": ["--width", str(W), "--height", str(H), "--samples", "2", "--pathtrace", "--bounces", "4"],
     "wh": ["--width", str(W), "--height", str(H), "--samples", "2", "--depth", "4"],
+    "mis": ["--width", str(W), "--height", str(H), "--samples", "2", "--pathtrace", "--mis", "--bounces", "4"],
 }
 SEED = 5
 

@@ -213,8 +213,8 @@ ADJ = np.load(golden("adj_cornell.npz"))
 def _adj_scene(mode):
     hs = pa.HostScene.from_pecl(golden("adj_init.pecl"), int(ADJ["w"]), int(ADJ["h"]), 2, 4)
     s = hs.settings()
-    if mode == "pt":
-        s.pathtrace, s.pathtrace_bounces = 1, 4
+    if mode in ("pt", "mis"):
+        s.pathtrace, s.pathtrace_bounces = (1 if mode == "pt" else 2), 4
     else:
         s.pathtrace, s.max_depth = 0, 4
     return hs, s
@@ -225,7 +225,7 @@ def _oracle_loss(hs, s, target):
     return float(np.sum((img.astype(np.float64) - target) ** 2))
 
 
-@pytest.mark.parametrize("mode", ["pt", "wh"])
+@pytest.mark.parametrize("mode", ["pt", "wh", "mis"])
 def test_adjoint_fixture_loss_and_fd_match_oracle(mode):
     """The fixture's loss and a few of its central differences, recomputed
     through the oracle (the same renders, parameters edited on the host)."""
@@ -256,3 +256,11 @@ def test_adjoint_fixture_structure():
     node's local term is discarded): the reference's FD reads zero there."""
     assert np.abs(ADJ["fd_wh"][7, 3:]).max() == 0.0
     assert np.abs(ADJ["fd_pt"][7, 3:]).max() <= 2e-3   # the FD noise floor of a ~5e3 loss
+    assert np.abs(ADJ["fd_mis"][7, 3:]).max() <= 2e-3
+    # MIS reweights the light term only (a point light here: w_light near 1),
+    # so its differences stay near the plain path tracer's, but its albedo
+    # entries move by more than the gradient tests' bar (0.25 + 1e-3 |fd|):
+    # an adjoint that ignored the MIS weights would fail them
+    assert np.allclose(ADJ["fd_mis"], ADJ["fd_pt"], rtol=0.05, atol=5.0)
+    d = np.abs(ADJ["fd_mis"][:, :3] - ADJ["fd_pt"][:, :3])
+    assert (d > 0.25 + 1e-3 * np.abs(ADJ["fd_mis"][:, :3])).sum() >= 3
