@@ -211,6 +211,46 @@ def test_torch_autograd_function(core, case):
     assert np.array_equal(params.grad.numpy(), case["grad"])
 
 
+# A lens camera (aperture, polygonal bokeh) spends RNG draws on every primary
+# ray: the pass must replay them to render the forward image and follow the
+# same paths.  Emission entries against exact central differences.
+@pytest.mark.parametrize("mode", MODES)
+def test_lens_camera(core, mode):
+    hs, st = scene(mode)
+    cam = hs.camera()
+    cam.aperture, cam.focus_dist, cam.bokeh_blades = 0.15, 5.0, 6
+    sid = core.upload(hs.desc())
+    mats, n = pa.materials_of(hs.desc())
+    tgt = FIX["target_" + mode]
+    try:
+        g, loss, img, _ = core.adjoint(sid, cam, st, tgt)
+        ref, _ = core.render(sid, cam, st)
+        assert np.array_equal(img, ref)
+        plain, _ = core.render(sid, hs.camera(), st)
+        assert not np.array_equal(img, plain)   # the lens changed the image
+        d = (ref - tgt).astype(np.float64)   # the residual in float32, squared and summed in FP64
+        assert abs(loss - float(np.sum(d * d))) <= 1e-9 * loss
+        base = pa.material_params(mats, n)
+        t64 = tgt.astype(np.float64)
+        try:
+            for m, k in [(0, 3), (2, 4), (4, 5), (6, 3)]:
+                vals = []
+                for sgn in (1, -1):
+                    p = base.copy()
+                    p[m, k] = np.float32(base[m, k]) + sgn * np.float32(0.5)
+                    pa.set_material_params(mats, n, p)
+                    core.set_materials(sid, mats, n)
+                    im, _ = core.render(sid, cam, st)
+                    vals.append(float(np.sum((im.astype(np.float64) - t64) ** 2)))
+                fd = (vals[0] - vals[1]) / 1.0
+                assert abs(g[m, k] - fd) <= 1e-5 * abs(fd) + 1e-3, (m, k, g[m, k], fd)
+        finally:
+            pa.set_material_params(mats, n, base)
+            core.set_materials(sid, mats, n)
+    finally:
+        core.free_scene(sid)
+
+
 def test_limits(core):
     hs = pa.HostScene.stress(freq=4, n_spheres=80, seed=1, width=16, height=16)   # 81 materials
     desc = hs.desc()
