@@ -40,8 +40,9 @@ __device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
 // The pixel's adjoint weight W is then known: given (adj, the VJP), or the L2
 // loss's 2 (I - target), whose loss is reduced into the last slot.  W scales J
 // once: the gradient is the sum over pixels of W_c J / n^2.
-// 128 VGPRs, 4 waves per SIMD: the 512^2 64-spp gradient step measured
-// 85.7 -> 77.8 ms against the unconstrained 208 VGPRs (2 waves).
+// 128 VGPRs, 4 waves per SIMD: on the 512^2 64-spp, 8-material step the
+// pass measured 37.2 ms, against 42.5 ms at 3 waves and 45.3 ms at 5; 8x8
+// tiles and 64-pixel row segments per block measured the same.
 template <int NM>
 __global__ __launch_bounds__(kAdjBlock, 4) void k_adjoint(DevScene S, CamRec C, RSettings st, const float* tgt,
                                                        const float* adj, float* img_out, double* partial,
