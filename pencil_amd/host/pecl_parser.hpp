@@ -6,12 +6,14 @@
 // (:166-284); `<x, y, z>` vectors (:335-361); 0-100 colors (:405-419);
 // material patches applied onto the current material (:487-781); one
 // material per primitive block (:1619-1838); `render {}` settings
-// (:1268-1388).  Blocks whose features live outside the core (fog, sky,
-// terrain, implicit surfaces, textures, presets, dispersion, ...) are
-// recorded in ParsedScene::unsupported so the renderer can refuse them.
+// (:1268-1388); terrain grids of triangles (:1180-1207, :1541-1576).  Blocks
+// whose features live outside the core (fog, sky, implicit surfaces,
+// textures, presets, dispersion, ...) are recorded in
+// ParsedScene::unsupported so the renderer can refuse them.
 #pragma once
 
 #include "mesh.hpp"
+#include "noise.hpp"
 #include "scene.hpp"
 
 #include <cctype>
@@ -48,6 +50,9 @@ struct ParsedScene {
     // scratch fields reused while parsing each primitive block (:85-97)
     Vec3 _tmp_v1 = {0, 0, 0}, _tmp_v2 = {0, 0, 0}, _tmp_v3 = {0, 0, 0}, _tmp_v4 = {0, 0, 0};
     float _tmp_f1 = 1.0f, _tmp_f2 = 1.0f;
+    int _tmp_i1 = 0, _tmp_i2 = 32;   // terrain octaves / grid (:86-87); the grid survives the
+                                     // end-of-block reset (:1847-1851), the octaves do not
+    std::string _pending_seed;       // terrain seed text (:91), cleared by the terrain block
     std::string _pending_mesh_file, _pending_include, _base_dir;
     bool _skip_finalize = false;
     Material _current_mat = make_diffuse({0.7f, 0.7f, 0.7f}, 0.0f, 32.0f);
@@ -431,6 +436,22 @@ inline bool parse_pecl_source(const std::string& src, ParsedScene& out, std::vec
                 else if (key == "rad" || key == "radius") consumed = parse_scalar(lx, out._tmp_f1, e);
                 else if (key == "h" || key == "height") consumed = parse_scalar(lx, out._tmp_f2, e);
             }
+            if (!consumed && (block == "tern" || block == "terrain")) {   // :1180-1207
+                if (key == "size") {
+                    int w, h;
+                    if (parse_size2(lx, w, h, e)) { out._tmp_v1 = {(float)w, 0, (float)h}; consumed = true; }
+                } else if (key == "height" || key == "h") consumed = parse_scalar(lx, out._tmp_f1, e);
+                else if (key == "scale" || key == "freq") consumed = parse_scalar(lx, out._tmp_f2, e);
+                else if (key == "octaves" || key == "oct") {
+                    float f; if (parse_scalar(lx, f, e)) { out._tmp_i1 = (int)f; consumed = true; }
+                } else if (key == "trgrid" || key == "grid") {
+                    float f; if (parse_scalar(lx, f, e)) { out._tmp_i2 = (int)f; consumed = true; }
+                } else if (key == "seed") {   // the token's text, a name or a number, is hashed
+                    Token v = lx.next();
+                    if (v.kind == TokKind::Ident || v.kind == TokKind::Number) out._pending_seed = v.text;
+                    consumed = true;
+                }
+            }
             if (!consumed && block == "mesh") {
                 if (key == "file" || key == "path" || key == "obj") { out._pending_mesh_file = lx.read_path(); consumed = true; }
                 else if (key == "scale" || key == "fit") consumed = parse_scalar(lx, out._tmp_f1, e);
@@ -575,8 +596,34 @@ inline bool parse_pecl_source(const std::string& src, ParsedScene& out, std::vec
             if (out._tmp_f1 != 1.0f) out.unsupported.push_back("group scale");
         } else if (block == "include" || block == "render") {
             // include handled below; render consumed into fields
-        } else if (block == "fog" || block == "sky" || block == "skybox" || block == "env" || block == "tern" ||
-                   block == "terrain" || block == "gsky" || block == "gen_sky" || block == "surf") {
+        } else if (block == "tern" || block == "terrain") {
+            // :1541-1576: a grid x grid field of quads over [-w/2, w/2] x [-d/2, d/2],
+            // each split into two triangles, heights fbm(x f, z f) * h
+            const int mid = add_material();
+            const float tw = out._tmp_v1.x > 0 ? out._tmp_v1.x : 20.0f;
+            const float td = out._tmp_v1.z > 0 ? out._tmp_v1.z : 20.0f;
+            const float th = out._tmp_f1 > 0 ? out._tmp_f1 : 3.0f;
+            const float freq = out._tmp_f2 > 0 ? out._tmp_f2 : 0.1f;
+            const int octaves = out._tmp_i1 > 0 ? out._tmp_i1 : 4;
+            const uint32_t seed = out._pending_seed.empty() ? (uint32_t)(12345 + mid * 7)
+                                                            : GradientNoise2::seed_of(out._pending_seed);
+            const GradientNoise2 noise(seed);
+            const int grid = out._tmp_i2 > 0 ? out._tmp_i2 : 32;
+            for (int zi = 0; zi < grid; ++zi) {
+                for (int xi = 0; xi < grid; ++xi) {
+                    const float x0 = (float)xi / grid * tw - tw / 2, z0 = (float)zi / grid * td - td / 2;
+                    const float x1 = (float)(xi + 1) / grid * tw - tw / 2, z1 = (float)(zi + 1) / grid * td - td / 2;
+                    const Point3 a{x0, noise.fbm(x0 * freq, z0 * freq, octaves) * th, z0};
+                    const Point3 b{x1, noise.fbm(x1 * freq, z0 * freq, octaves) * th, z0};
+                    const Point3 c{x0, noise.fbm(x0 * freq, z1 * freq, octaves) * th, z1};
+                    const Point3 d{x1, noise.fbm(x1 * freq, z1 * freq, octaves) * th, z1};
+                    out.scene.add_primitive(make_triangle(a, b, d, mid));
+                    out.scene.add_primitive(make_triangle(a, d, c, mid));
+                }
+            }
+            out._pending_seed.clear();
+        } else if (block == "fog" || block == "sky" || block == "skybox" || block == "env" ||
+                   block == "gsky" || block == "gen_sky" || block == "surf") {
             out.unsupported.push_back(block + " block");
         } else {
             errors.push_back({t.line, "unknown block '" + block + "' (skipped)"});
@@ -585,7 +632,7 @@ inline bool parse_pecl_source(const std::string& src, ParsedScene& out, std::vec
         if (!patch.outside.empty() && block != "mat") out.unsupported.push_back(patch.outside);
 
         out._tmp_v1 = {0, 0, 0}; out._tmp_v2 = {0, 0, 0}; out._tmp_v3 = {0, 0, 0}; out._tmp_v4 = {0, 0, 0};
-        out._tmp_f1 = 1.0f; out._tmp_f2 = 1.0f;
+        out._tmp_f1 = 1.0f; out._tmp_f2 = 1.0f; out._tmp_i1 = 0;
 
         if (!out._pending_include.empty()) {   // :1859-1883
             std::string inc = out._pending_include;

@@ -227,6 +227,24 @@ def test_render_matches_reference_hdr(core, fx_name):
         assert exact >= 0.99 and within >= 0.999
 
 
+def test_terrain_scene_matches_reference(core):
+    """Two noise terrains (1730 triangles) from the terrain block against the
+    reference's render of tests/golden/terrain.pecl.  The scene has glass, so
+    Schlick's pow applies the deterministic-scene bar: >= 99 % of pixels
+    exact, every channel within 1/255 after the 8-bit encode."""
+    T = np.load(golden("terrain_ref.npz"))
+    hs = pa.HostScene.from_pecl(golden("terrain.pecl"), int(T["w"]), int(T["h"]), 2, 3)
+    sid = core.upload(hs.desc())
+    try:
+        img, _ = core.render(sid, hs.camera(), hs.settings())
+    finally:
+        core.free_scene(sid)
+    exact, rmse = _stats(img, T["hdr"])
+    d8 = np.abs(pa.tonemap_rgb8(img).astype(int) - pa.tonemap_rgb8(T["hdr"]).astype(int))
+    print(f"terrain: exact={exact:.5f} rmse={rmse:.2e} max8={d8.max()}")
+    assert exact >= 0.99 and d8.max() <= 1
+
+
 @pytest.mark.parametrize("name", SCENES)
 def test_render_matches_reference_golden_png(core, name):
     hs = load(name)

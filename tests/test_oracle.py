@@ -264,3 +264,20 @@ def test_adjoint_fixture_structure():
     assert np.allclose(ADJ["fd_mis"], ADJ["fd_pt"], rtol=0.05, atol=5.0)
     d = np.abs(ADJ["fd_mis"][:, :3] - ADJ["fd_pt"][:, :3])
     assert (d > 0.25 + 1e-3 * np.abs(ADJ["fd_mis"][:, :3])).sum() >= 3
+
+
+def test_terrain_blocks_match_reference():
+    """Terrain blocks (tern, src/pecl_parser.hpp:1180-1207, 1541-1576), parsed
+    here and rendered by the oracle, equal the reference's render of the same
+    file through its own parser, bit for bit.  This covers the grid, the fBm
+    heights, a name seed and a number seed, and the end-of-block reset
+    (:1847-1851): the second block's scale and octaves fall back to
+    1.0 and 4, while its grid would survive.  tests/golden/make_terrain_fixture.py
+    wrote the fixture with the reference build."""
+    T = np.load(golden("terrain_ref.npz"))
+    hs = pa.HostScene.from_pecl(golden("terrain.pecl"), int(T["w"]), int(T["h"]), 2, 3)
+    d = hs.desc()
+    assert d.n_prims == 2 * 28 * 28 + 2 * 9 * 9 + 1 and hs.unsupported() == ""
+    img, _ = ol.OracleScene(d).render(hs.camera(), hs.settings())
+    assert np.array_equal(img, T["hdr"])
+
