@@ -55,11 +55,25 @@ struct V3 {
     V3 reflect(const V3& n) const { return *this - n * (2.0f * this->dot(n)); }
 };
 static inline float clampf(float v, float lo, float hi) { return (v < lo) ? lo : (v > hi) ? hi : v; }
+
+// Transcendentals of the shading path.  Mode 0 (default): the float libm
+// functions the reference calls (glibc cosf / sinf / powf / atan2f,
+// src/rng.hpp:42-43, src/vec3.hpp:86, src/renderer.hpp:564,941,959,
+// src/camera.cpp:123-139).  Mode 1: the same function evaluated in double and
+// rounded once to float, i.e. the correctly rounded result except in rare
+// double-rounding cases -- the arithmetic the HIP core uses.  Mode 1 lets a
+// test compare the core with this restatement bit for bit where glibc's
+// not-correctly-rounded float results would otherwise decorrelate a path.
+static int g_trig_mode = 0;
+static inline float o_cos(float x) { return g_trig_mode ? (float)std::cos((double)x) : std::cos(x); }
+static inline float o_sin(float x) { return g_trig_mode ? (float)std::sin((double)x) : std::sin(x); }
+static inline float o_pow(float x, float y) { return g_trig_mode ? (float)std::pow((double)x, (double)y) : std::pow(x, y); }
+static inline float o_atan2(float y, float x) { return g_trig_mode ? (float)std::atan2((double)y, (double)x) : std::atan2(y, x); }
 // vec3.hpp:83-87
 static inline float schlick(float c, float r) {
     float r0 = (1.0f - r) / (1.0f + r);
     r0 = r0 * r0;
-    return r0 + (1.0f - r0) * std::pow(1.0f - c, 5.0f);
+    return r0 + (1.0f - r0) * o_pow(1.0f - c, 5.0f);
 }
 
 // ---- rng.hpp:13-45 --------------------------------------------------------
@@ -76,8 +90,8 @@ struct Rng {
         float theta, r;
         if (std::fabs(sx) > std::fabs(sy)) { r = sx; theta = (3.14159265f / 4.0f) * (sy / sx); }
         else { r = sy; theta = (3.14159265f / 2.0f) - (3.14159265f / 4.0f) * (sx / sy); }
-        dx = r * std::cos(theta);
-        dy = r * std::sin(theta);
+        dx = r * o_cos(theta);
+        dy = r * o_sin(theta);
     }
 };
 
@@ -590,12 +604,12 @@ struct Camera {
             rng.unit_disc(dx, dy);
             float angle_step = 2.0f * 3.14159265358979f / blades;
             float r = std::sqrt(dx * dx + dy * dy);
-            float theta = std::atan2(dy, dx);
+            float theta = o_atan2(dy, dx);
             if (theta < 0) theta += 2.0f * 3.14159265358979f;
             int sector = (int)(theta / angle_step);
             float local_angle = theta - sector * angle_step;
             float half_step = angle_step * 0.5f;
-            float max_r = std::cos(half_step) / std::cos(local_angle - half_step);
+            float max_r = o_cos(half_step) / o_cos(local_angle - half_step);
             if (r <= max_r) return;
         }
         rng.unit_disc(dx, dy);
@@ -691,7 +705,7 @@ struct Renderer {
                     V3 Vv = -r.d.normalized();
                     V3 H = (Ld + Vv).normalized();
                     float ndoth = clampf(shade_normal.dot(H), 0.0f, 1.0f);
-                    float sp = std::pow(ndoth, m.shin);
+                    float sp = o_pow(ndoth, m.shin);
                     c += m.spec_color * L.I * (sp * m.spec);
                 }
                 contrib += c;
@@ -806,7 +820,7 @@ struct Renderer {
                 V3 t2 = refl.cross(t1).normalized();
                 float r1 = rng.unit() * 0.1f;
                 float r2 = rng.unit() * 6.28f;
-                nd = (refl + t1 * (r1 * std::cos(r2)) + t2 * (r1 * std::sin(r2))).normalized();
+                nd = (refl + t1 * (r1 * o_cos(r2)) + t2 * (r1 * o_sin(r2))).normalized();
                 NT = T * albedo * m.refl;
             } else if (m.kind == PCL_MAT_DIELECTRIC) {
                 V3 g = trace_dielectric(cur, rec, m, albedo, depth, rng);
@@ -820,7 +834,7 @@ struct Renderer {
                 V3 up = (std::fabs(N.y) < 0.9f) ? V3{0, 1, 0} : V3{1, 0, 0};
                 V3 Tt = up.cross(N).normalized();
                 V3 B = N.cross(Tt).normalized();
-                nd = (Tt * std::cos(r1) * r2s + B * std::sin(r1) * r2s + N * std::sqrt(1.0f - r2)).normalized();
+                nd = (Tt * o_cos(r1) * r2s + B * o_sin(r1) * r2s + N * std::sqrt(1.0f - r2)).normalized();
             }
             cur = Ray{rec.p + rec.normal * st.shadow_bias, nd};
             T = NT;
@@ -902,6 +916,15 @@ typedef struct orc_counts {
     uint64_t node_visits;
     uint64_t prim_tests[6];
 } orc_counts;
+
+// 0: libm float transcendentals (the reference's); 1: double-evaluated,
+// rounded once (the HIP core's).  Returns the previous mode.  Process-wide;
+// set it before rendering, not while another thread renders.
+int orc_set_trig_mode(int mode) {
+    const int prev = orc::g_trig_mode;
+    orc::g_trig_mode = mode ? 1 : 0;
+    return prev;
+}
 
 void* orc_scene_create(const pcl_scene_desc* d) {
     auto* h = new orc::SceneHandle();

@@ -15,15 +15,18 @@
 //
 // Usage:
 //   ref_harness render <scene.pecl> <out.bin> [opts]   HDR float32 W*H*3 + u64 rays
+//                       (--rows r0 r1: only rows [r0, r1) through the public
+//                       render_tile_region, header W, r1-r0)
 //   ref_harness hits   <scene.pecl> <W> <H> <out.bin>  pixel-center primary hits
 //   ref_harness bvh    <scene.pecl> <out.bin>          leaf order + split points
 //   ref_harness loss   <scene.pecl> <target.bin> <out.bin> [opts]
 //                       L2 loss and central FD of it over albedo/emission
 // opts: --width W --height H --samples n --depth d --pathtrace --mis --bounces b
-//       --threads T --h step --no-fd
+//       --threads T --h step --no-fd --rows r0 r1
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -61,6 +64,7 @@ struct Tagged : Primitive {
 
 struct Opts {
     int width = 0, height = 0, samples = 0, depth = 0, threads = 0, bounces = 4;
+    int row0 = -1, row1 = -1;
     bool pathtrace = false, fd = true, mis = false;
     float h = 1e-3f;
 };
@@ -80,6 +84,7 @@ static Opts parse_opts(int argc, char** argv, int from) {
         else if (s == "--mis") o.mis = true;
         else if (s == "--h") o.h = (float)atof(nxt());
         else if (s == "--no-fd") o.fd = false;
+        else if (s == "--rows") { o.row0 = atoi(nxt()); o.row1 = atoi(nxt()); }
     }
     return o;
 }
@@ -126,6 +131,37 @@ static int cmd_render(int argc, char** argv) {
     RenderSettings rs = settings_for(ps, o);
     Renderer r(ps.scene, ps.camera, rs);
     std::vector<Color> px; RenderStats st;
+    if (o.row0 >= 0) {
+        // A band of rows, 16-pixel column tiles handed out to the threads; a
+        // pixel's value does not depend on the tile it is rendered in (its RNG
+        // is seeded from (i, j), src/renderer.hpp:313).
+        const int W = rs.width, H = rs.height;
+        const int r0 = std::max(0, o.row0), r1 = std::min(H, o.row1);
+        if (r1 <= r0) return 2;
+        px.assign((size_t)W * H, Color{0, 0, 0});
+        r.ray_counter_.store(0);
+        std::atomic<int> next{0};
+        const int tiles = (W + 15) / 16;
+        int T = o.threads > 0 ? o.threads : (int)std::thread::hardware_concurrency();
+        if (T <= 0) T = 1;
+        std::vector<std::thread> ws;
+        auto t0 = std::chrono::steady_clock::now();
+        for (int t = 0; t < T; ++t)
+            ws.emplace_back([&]() {
+                for (int k; (k = next.fetch_add(1)) < tiles;)
+                    r.render_tile_region(px, W, H, k * 16, r0, std::min(16, W - k * 16), r1 - r0);
+            });
+        for (auto& w : ws) w.join();
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::remove(argv[3]);
+        int32_t hdr[2] = {W, r1 - r0};
+        write_all(argv[3], hdr, sizeof(hdr));
+        write_all(argv[3], px.data() + (size_t)r0 * W, (size_t)(r1 - r0) * W * sizeof(Color));
+        uint64_t rays = r.ray_counter_.load();
+        write_all(argv[3], &rays, sizeof(rays));
+        std::cout << "rays " << rays << "\nseconds " << sec << "\n";
+        return 0;
+    }
     r.render(px, &st);
     std::remove(argv[3]);
     int32_t hdr[2] = {rs.width, rs.height};

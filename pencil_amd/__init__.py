@@ -171,10 +171,15 @@ def host_lib():
         _sig(lib, "pclh_load_pecl", _P, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int])
         _sig(lib, "pclh_make_stress", _P, [C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int])
         _sig(lib, "pclh_write_stress", C.c_int,
-             [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_char_p, C.c_int])
+             [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int])
+        _sig(lib, "pclh_write_cornell_mesh", C.c_int,
+             [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float,
+              C.c_char_p, C.c_int])
         _sig(lib, "pclh_make_cornell_mesh", _P, [C.c_char_p, C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_float,
                                                   C.c_float, C.c_int, C.c_int, C.c_char_p, C.c_int])
         _sig(lib, "pclh_make_mixed", _P, [C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int])
+        _sig(lib, "pclh_renderer_render", C.c_int,
+             [_P, C.POINTER(Settings), C.c_int, _P, C.POINTER(C.c_uint64), C.c_char_p, C.c_int])
         _sig(lib, "pclh_free", None, [_P])
         _sig(lib, "pclh_desc", C.c_int, [_P, C.POINTER(SceneDesc)])
         _sig(lib, "pclh_camera", C.c_int, [_P, C.POINTER(Camera)])
@@ -227,10 +232,21 @@ class HostScene:
         return cls(host_lib().pclh_make_mixed(freq, n_spheres, seed, width, height))
 
     @staticmethod
-    def write_stress(directory: str, name: str, freq=229, n_spheres=4096, seed=1234, width=1920, height=1080):
+    def write_stress(directory: str, name: str, freq=229, n_spheres=4096, seed=1234, width=1920, height=1080,
+                     mat_mix=0):
+        """The stress scene (mat_mix=1: a batched-workload scene, as `mixed`) as .pecl + .obj files."""
         err = C.create_string_buffer(1024)
         if host_lib().pclh_write_stress(directory.encode(), name.encode(), freq, n_spheres, seed, width, height,
-                                        err, len(err)) != 0:
+                                        mat_mix, err, len(err)) != 0:
+            raise RuntimeError(err.value.decode())
+
+    @staticmethod
+    def write_cornell_mesh(directory: str, name: str, pecl: str, freq=229, seed=1234, scale=0.25,
+                           center=(0.0, 0.8, 0.3)):
+        """`cornell_mesh`'s scene as .pecl + .obj files the reference parses."""
+        err = C.create_string_buffer(1024)
+        if host_lib().pclh_write_cornell_mesh(pecl.encode(), directory.encode(), name.encode(), freq, seed, scale,
+                                              *center, err, len(err)) != 0:
             raise RuntimeError(err.value.decode())
 
     def desc(self) -> SceneDesc:
@@ -247,6 +263,17 @@ class HostScene:
         s = Settings()
         host_lib().pclh_settings(self._h, C.byref(s))
         return s
+
+    def render_api(self, settings: Settings = None, bokeh_blades: int = 0):
+        """Renders through pencil::Renderer::render (the drop-in entry point,
+        backend hip): (W x H x 3 float32 image, (rays, closest, any-hit))."""
+        s = settings if settings is not None else self.settings()
+        img = np.empty((s.height, s.width, 3), dtype=np.float32)
+        counts = (C.c_uint64 * 3)()
+        err = C.create_string_buffer(4096)
+        if host_lib().pclh_renderer_render(self._h, C.byref(s), bokeh_blades, _ptr(img), counts, err, len(err)) != 0:
+            raise RuntimeError("Renderer::render failed: " + err.value.decode())
+        return img, tuple(int(x) for x in counts)
 
     def unsupported(self) -> str:
         buf = C.create_string_buffer(4096)

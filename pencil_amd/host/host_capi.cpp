@@ -62,10 +62,12 @@ void* pclh_make_stress(int freq, int n_spheres, uint32_t seed, int width, int he
     return h.release();
 }
 
+// mat_mix = 1 writes the batched workload's scene (pclh_make_mixed).
 int pclh_write_stress(const char* dir, const char* name, int freq, int n_spheres, uint32_t seed, int width,
-                      int height, char* err, int errlen) {
+                      int height, int mat_mix, char* err, int errlen) {
     pencil::StressParams P;
     P.freq = freq; P.n_spheres = n_spheres; P.seed = seed; P.width = width; P.height = height;
+    P.mat_mix = mat_mix;
     std::string e;
     if (!pencil::write_stress_scene(P, dir, name, e)) { copy_err(e, err, errlen); return -1; }
     return 0;
@@ -86,6 +88,18 @@ void* pclh_make_cornell_mesh(const char* pecl, int freq, uint32_t seed, float sc
     return h;
 }
 
+// pclh_make_cornell_mesh's scene as files the reference parses: <dir>/<name>.pecl
+// (the base .pecl plus a mesh{} block) and <dir>/<name>.obj.
+int pclh_write_cornell_mesh(const char* pecl, const char* dir, const char* name, int freq, uint32_t seed,
+                            float scale, float cx, float cy, float cz, char* err, int errlen) {
+    pencil::StressParams P;
+    P.freq = freq; P.seed = seed;
+    P.mesh_scale = scale; P.mesh_offset[0] = cx; P.mesh_offset[1] = cy; P.mesh_offset[2] = cz;
+    std::string e;
+    if (!pencil::write_mesh_in_scene(P, pecl, dir, name, e)) { copy_err(e, err, errlen); return -1; }
+    return 0;
+}
+
 // One scene of the batched workload: a displaced icosphere of frequency freq
 // (radius 3) and n_spheres seeded spheres with a diffuse / metal / glass mix,
 // the stress scene's camera and a soft point light.
@@ -97,6 +111,29 @@ void* pclh_make_mixed(int freq, int n_spheres, uint32_t seed, int width, int hei
     h->ps.scene.lights_mut()[0].radius = 0.5f;
     h->refresh();
     return h.release();
+}
+
+// The drop-in entry itself: pencil::Renderer::render (host/renderer.hpp, the
+// reference's src/renderer.hpp:198 signature) on the scene with settings s
+// and an optional --bokeh override, backend hip on device 0.  out_rgb:
+// W*H*3 floats.  Returns 0, or -1 with the renderer's message in err.
+int pclh_renderer_render(void* hv, const pcl_settings* s, int bokeh_blades, float* out_rgb, uint64_t* counts,
+                         char* err, int errlen) {
+    auto* h = static_cast<Handle*>(hv);
+    if (!h || !s || !out_rgb) return -1;
+    pencil::RenderSettings rs;
+    rs.width = s->width; rs.height = s->height; rs.samples = s->samples; rs.max_depth = s->max_depth;
+    rs.shadow_bias = s->shadow_bias; rs.soft_shadow_samples = s->soft_shadow_samples;
+    rs.pathtrace = s->pathtrace != 0; rs.mis = s->pathtrace == 2; rs.pathtrace_bounces = s->pathtrace_bounces;
+    rs.bokeh_blades = bokeh_blades;
+    rs.backend = pencil::Backend::Hip;
+    pencil::Renderer r(h->ps.scene, h->ps.camera, rs);
+    std::vector<pencil::Color> px;
+    pencil::RenderStats st;
+    if (!r.render(px, &st)) { copy_err(r.error(), err, errlen); return -1; }
+    std::memcpy(out_rgb, px.data(), px.size() * sizeof(pencil::Color));
+    if (counts) { counts[0] = st.rays_traced; counts[1] = st.closest_queries; counts[2] = st.anyhit_queries; }
+    return 0;
 }
 
 void pclh_free(void* h) { delete static_cast<Handle*>(h); }

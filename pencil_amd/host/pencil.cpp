@@ -33,6 +33,7 @@ void usage() {
                  "  --pathtrace        path tracing with next-event estimation\n"
                  "  --mis              with --pathtrace: MIS-weighted light samples (trace_path_mis)\n"
                  "  --no-bvh           linear primitive scan\n"
+                 "  --bokeh N          polygonal aperture with N blades (0 = the scene's)\n"
                  "  --exposure F --tonemap 0|1|2 --quiet\n";
 }
 
@@ -40,7 +41,7 @@ void usage() {
 
 int main(int argc, char** argv) {
     std::string scene_path, out_path, backend = "hip";
-    int ow = 0, oh = 0, os = 0, od = 0, bounces = 4, device = 0, tonemap = 1;
+    int ow = 0, oh = 0, os = 0, od = 0, bounces = 4, device = 0, tonemap = 1, bokeh = 0;
     bool quiet = false, pathtrace = false, no_bvh = false, mis = false;
     float exposure = 0.0f;
     for (int i = 1; i < argc; ++i) {
@@ -61,6 +62,7 @@ int main(int argc, char** argv) {
         else if (s == "--pathtrace" || s == "--pt") pathtrace = true;
         else if (s == "--mis") mis = true;
         else if (s == "--no-bvh" || s == "--nobvh") no_bvh = true;
+        else if (s == "--bokeh") bokeh = std::atoi(next());   // src/pencil.cpp:218
         else if (s == "--quiet") quiet = true;
         else if (s == "--exposure") exposure = std::strtof(next(), nullptr);
         else if (s == "--tonemap") tonemap = std::atoi(next());
@@ -103,6 +105,7 @@ int main(int argc, char** argv) {
     rs.pathtrace = pathtrace || ps.enable_pathtrace;
     rs.mis = mis || ps.enable_mis;   // src/pencil.cpp:749
     rs.pathtrace_bounces = bounces;
+    rs.bokeh_blades = bokeh;
     rs.backend = pencil::Backend::Hip;
     rs.device = device;
     if (!quiet) {

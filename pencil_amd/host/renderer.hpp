@@ -33,6 +33,7 @@ struct RenderSettings {
     bool mis = false;            // trace_path_mis instead of trace_path (src/renderer.hpp:55)
     int pathtrace_bounces = 4;
     int tile_size = 64;
+    int bokeh_blades = 0;        // > 0 overrides the camera's aperture shape (src/renderer.hpp:162-167)
     Backend backend = Backend::Hip;
     int device = 0;
 };
@@ -49,7 +50,10 @@ struct RenderStats {
 class Renderer {
 public:
     Renderer(const Scene& scene, const Camera& camera, RenderSettings s)
-        : scene_(scene), camera_(camera), settings_(s) {}
+        : scene_(scene), camera_(camera), settings_(s) {
+        // a CLI --bokeh N wins over the scene's cmra { bd: N }; 0 keeps the scene's
+        if (settings_.bokeh_blades > 0) camera_.set_bokeh_blades(settings_.bokeh_blades);
+    }
     ~Renderer() {
         if (ctx_) {
             if (scene_id_ >= 0) pcl_scene_free(ctx_, scene_id_);

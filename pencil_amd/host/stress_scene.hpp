@@ -259,11 +259,51 @@ inline bool write_stress_scene(const StressParams& P, const std::string& dir, co
     std::fprintf(f, "cmra {\n  pos: <0, 4, -18>\n  aim: <0, 0, 0>\n  up: <0, 1, 0>\n  fov: 50\n}\n");
     std::fprintf(f, "mat { clr: <70, 70, 70> }\nmesh {\n  file: %s\n  scale: 200\n}\n", obj.c_str());
     for (const auto& s : stress::spheres(P)) {
-        std::fprintf(f, "mat { clr: <%d, %d, %d> }\n", s.clr[0], s.clr[1], s.clr[2]);
+        if (s.kind == 1)
+            std::fprintf(f, "mat { clr: <%d, %d, %d>, fuz: %d }\n", s.clr[0], s.clr[1], s.clr[2], s.fuz);
+        else if (s.kind == 2)
+            std::fprintf(f, "mat { clr: <%d, %d, %d>, ior: 1.5 }\n", s.clr[0], s.clr[1], s.clr[2]);
+        else
+            std::fprintf(f, "mat { clr: <%d, %d, %d> }\n", s.clr[0], s.clr[1], s.clr[2]);
         std::fprintf(f, "sphere { pos: <%.9g, %.9g, %.9g>, rad: %.9g }\n", s.c.x, s.c.y, s.c.z, s.r);
     }
-    std::fprintf(f, "light {\n  type: point\n  pos: <10, 15, -10>\n  clr: <100, 100, 100>\n  radius: 0\n}\n");
+    // the batched workload's scenes (mat_mix) use a soft light of radius 0.5
+    std::fprintf(f, "light {\n  type: point\n  pos: <10, 15, -10>\n  clr: <100, 100, 100>\n  radius: %s\n}\n",
+                 P.mat_mix ? "0.5" : "0");
     std::fprintf(f, "render {\n  size: %d %d\n  spp: 1\n  depth: 1\n}\n", P.width, P.height);
+    std::fclose(f);
+    return true;
+}
+
+// Writes <dir>/<name>.pecl -- the text of the .pecl at `base` followed by a
+// diffuse 0.7 material and a mesh{} block -- and <dir>/<name>.obj with the
+// displaced icosphere placed as add_mesh places it (v * mesh_scale +
+// mesh_offset), plus the +-100 guard vertices, so the reference parses the
+// scene add_mesh builds on top of `base`, float for float.
+inline bool write_mesh_in_scene(const StressParams& P, const std::string& base, const std::string& dir,
+                                const std::string& name, std::string& err) {
+    std::vector<Vec3> verts;
+    std::vector<int32_t> tris;
+    stress::icosphere(P, verts, tris);
+    const std::string obj = dir + "/" + name + ".obj", pecl = dir + "/" + name + ".pecl";
+    FILE* in = std::fopen(base.c_str(), "rb");
+    if (!in) { err = "cannot read " + base; return false; }
+    std::string text;
+    char buf[4096];
+    for (size_t n; (n = std::fread(buf, 1, sizeof(buf), in)) > 0;) text.append(buf, n);
+    std::fclose(in);
+    FILE* f = std::fopen(obj.c_str(), "w");
+    if (!f) { err = "cannot write " + obj; return false; }
+    std::fprintf(f, "# displaced icosphere, frequency %d, seed %u\n", P.freq, P.seed);
+    for (const auto& v : verts) std::fprintf(f, "v %.9g %.9g %.9g\n", v.x, v.y, v.z);
+    std::fprintf(f, "v -100 -100 -100\nv 100 100 100\n");
+    for (size_t t = 0; t + 2 < tris.size(); t += 3)
+        std::fprintf(f, "f %d %d %d\n", tris[t] + 1, tris[t + 1] + 1, tris[t + 2] + 1);
+    std::fclose(f);
+    f = std::fopen(pecl.c_str(), "w");
+    if (!f) { err = "cannot write " + pecl; return false; }
+    std::fwrite(text.data(), 1, text.size(), f);
+    std::fprintf(f, "\nmat { clr: <70, 70, 70> }\nmesh {\n  file: %s\n  scale: 200\n}\n", obj.c_str());
     std::fclose(f);
     return true;
 }

@@ -1,3 +1,4 @@
+import json
 import os
 import sys
 
@@ -21,6 +22,20 @@ def golden(name):
     return os.path.join(GOLDEN, name)
 
 
+def img_stats(img, ref):
+    """Parity figures of an HDR image against a reference: fraction of pixels
+    bit-identical in all channels, RMSE, and the largest 8-bit difference and
+    fraction of pixels within one 8-bit step after the CLI's encode (ACES,
+    gamma 2.2)."""
+    import numpy as np
+    import pencil_amd as pa
+    exact = float(np.all(img == ref, axis=-1).mean())
+    rmse = float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+    d8 = np.abs(pa.tonemap_rgb8(img).astype(int) - pa.tonemap_rgb8(ref).astype(int))
+    return {"exact": exact, "rmse": rmse, "max8": int(d8.max()) if d8.size else 0,
+            "within1": float((d8 <= 1).all(-1).mean()) if d8.size else 1.0}
+
+
 @pytest.fixture(scope="session")
 def core():
     """One pcl_ctx on device 0 for the whole GPU session."""
@@ -28,3 +43,28 @@ def core():
     c = pa.Core(0)
     yield c
     c.close()
+
+
+_PARITY = []
+
+
+@pytest.fixture(scope="session")
+def parity_log():
+    """Collects per-case parity figures (exact fraction, RMSE, max 8-bit
+    difference ...).  With PCL_PARITY_OUT set, they are written there as JSON
+    at the end of the session."""
+    def record(case, **figures):
+        row = {"case": case}
+        row.update({k: (float(v) if isinstance(v, (float, int)) and not isinstance(v, bool) else v)
+                    for k, v in figures.items()})
+        _PARITY.append(row)
+        print(json.dumps(row), flush=True)
+    return record
+
+
+def pytest_sessionfinish(session, exitstatus):
+    out = os.environ.get("PCL_PARITY_OUT")
+    if out and _PARITY:
+        os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+        with open(out, "w") as f:
+            json.dump(_PARITY, f, indent=1)

@@ -47,12 +47,30 @@ def lib():
             ("orc_tonemap_rgb8", C.c_int, [P, C.c_int64, C.c_float, P]),
             ("orc_rng_u32", C.c_uint32, [C.c_uint32, C.c_int32]),
             ("orc_check_thresholds", C.c_int64, [P, C.c_int32]),
+            ("orc_set_trig_mode", C.c_int, [C.c_int]),
         ]:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
         _lib = L
     return _lib
+
+
+class trig_mode:
+    """Context manager: 1 = the oracle evaluates cos / sin / pow / atan2 in
+    double and rounds once (the HIP core's arithmetic), 0 = the float libm
+    functions the reference calls (the default)."""
+
+    def __init__(self, mode):
+        self.mode = mode
+
+    def __enter__(self):
+        self.prev = lib().orc_set_trig_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_trig_mode(self.prev)
+        return False
 
 
 def _p(a):

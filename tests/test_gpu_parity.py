@@ -1,15 +1,20 @@
 """HIP core vs the reference (harness fixtures) and the pinned oracle.
 
-Bars (from the task's parity contract):
+Bars (the parity contract, with the figures measured for them):
   * primitive ids bit-exact; t within 1e-5 relative (observed: bit-exact);
-  * deterministic Whitted scenes within 1/255 per channel after ACES + gamma;
-    hard-shadow scenes are HDR bit-exact.  Scenes with soft shadows or depth of
-    field use cos/sin: the device evaluates them in double and rounds once,
-    the reference's glibc cosf/sinf differ from that in ~5e-4 of arguments by
-    one ulp, which can flip a penumbra sub-sample; those scenes must be HDR
-    exact on >= 99% of pixels and within 1/255 on >= 99.9%;
-  * path tracing: HDR exact on >= 95% of pixels, RMSE <= 0.01 against the
-    reference at the same spp and RNG streams.
+  * deterministic Whitted scenes: every channel of every pixel within 1/255
+    after the CLI's 8-bit encode (ACES + gamma), and >= 99 % of pixels HDR
+    bit-exact; scenes that call no cos/sin/pow (hard_shadows) HDR bit-exact.
+    Soft shadows, depth of field, Phong highlights and Schlick call cos, sin
+    and pow: the device evaluates them in double and rounds once, glibc's
+    cosf / sinf / powf differ from that by one ulp on ~5e-4 of arguments
+    (oracle with the device's rounding vs the reference fixtures: >= 99.76 %
+    of pixels exact, max 8-bit difference 0);
+  * path tracing at the fixture sizes (<= 96^2, <= 8 bounces, 4 spp): the
+    same 8-bit bar, >= 99 % of pixels exact and RMSE <= 1e-6 (measured with
+    the device's rounding: >= 99.73 % exact, RMSE <= 7.5e-8);
+  * against the oracle run with the device's transcendental rounding
+    (oracle_lib.trig_mode(1)) the core is bit-exact.
 """
 import json
 
@@ -19,7 +24,7 @@ from PIL import Image
 
 import pencil_amd as pa
 import oracle_lib as ol
-from conftest import SCENES, golden
+from conftest import SCENES, golden, img_stats
 from test_oracle import HDR_FIXTURES, render_fixture_inputs
 
 pytestmark = pytest.mark.gpu
@@ -200,34 +205,35 @@ def test_stress_full_scene_matches_reference_and_oracle(core):
         core.free_scene(sid)
 
 
-def _stats(a, b):
-    exact = np.all(a == b, axis=-1).mean()
-    return exact, float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
+def assert_reference_bar(name, f, path=False):
+    """The deterministic-scene bar (and the path-tracing one at fixture sizes)."""
+    assert f["max8"] <= 1, f"{name}: an 8-bit channel differs by {f['max8']}"
+    assert f["exact"] >= 0.99, f"{name}: only {f['exact']:.4f} of pixels HDR-exact"
+    if path:
+        assert f["rmse"] <= 1e-6, f"{name}: RMSE {f['rmse']:.3g}"
 
 
 @pytest.mark.parametrize("fx_name", HDR_FIXTURES)
-def test_render_matches_reference_hdr(core, fx_name):
+def test_render_matches_reference_hdr(core, fx_name, parity_log):
     fx, hs, s = render_fixture_inputs(fx_name)
     sid = core.upload(hs.desc())
     try:
         img, st = core.render(sid, hs.camera(), s)
     finally:
         core.free_scene(sid)
-    ref = fx["hdr"]
-    exact, rmse = _stats(img, ref)
-    d8 = np.abs(pa.tonemap_rgb8(img).astype(int) - pa.tonemap_rgb8(ref).astype(int))
-    within = (d8 <= 1).all(-1).mean()
-    print(f"{fx_name}: exact={exact:.5f} rmse={rmse:.2e} max8={d8.max()} within1={within:.5f} "
-          f"rays={st.rays_traced}/{int(fx['rays'])}")
-    if fx_name.startswith(("pt_", "mis_")):
-        assert exact >= 0.95 and rmse <= 1e-2
-    elif TRIG_FREE.get(fx_name, False):
-        assert exact == 1.0 and st.rays_traced == int(fx["rays"])
-    else:
-        assert exact >= 0.99 and within >= 0.999
+    f = img_stats(img, fx["hdr"])
+    with ol.trig_mode(1):
+        oimg, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    fo = img_stats(img, oimg)
+    parity_log(f"hdr_{fx_name} vs reference", rays=st.rays_traced, ref_rays=int(fx["rays"]), **f)
+    parity_log(f"hdr_{fx_name} vs oracle (device rounding)", **fo)
+    assert fo["exact"] == 1.0 and st.rays_traced == oc.rays_traced
+    if TRIG_FREE.get(fx_name, False):
+        assert f["exact"] == 1.0 and st.rays_traced == int(fx["rays"])
+    assert_reference_bar(fx_name, f, path=fx_name.startswith(("pt_", "mis_")))
 
 
-def test_terrain_scene_matches_reference(core):
+def test_terrain_scene_matches_reference(core, parity_log):
     """Two noise terrains (1730 triangles) from the terrain block against the
     reference's render of tests/golden/terrain.pecl.  The scene has glass, so
     Schlick's pow applies the deterministic-scene bar: >= 99 % of pixels
@@ -239,14 +245,17 @@ def test_terrain_scene_matches_reference(core):
         img, _ = core.render(sid, hs.camera(), hs.settings())
     finally:
         core.free_scene(sid)
-    exact, rmse = _stats(img, T["hdr"])
-    d8 = np.abs(pa.tonemap_rgb8(img).astype(int) - pa.tonemap_rgb8(T["hdr"]).astype(int))
-    print(f"terrain: exact={exact:.5f} rmse={rmse:.2e} max8={d8.max()}")
-    assert exact >= 0.99 and d8.max() <= 1
+    f = img_stats(img, T["hdr"])
+    parity_log("terrain vs reference", **f)
+    assert_reference_bar("terrain", f)
 
 
 @pytest.mark.parametrize("name", SCENES)
-def test_render_matches_reference_golden_png(core, name):
+def test_render_matches_reference_golden_png(core, name, parity_log):
+    """The reference's golden PNGs: every channel within one 8-bit step, and
+    identical for the scene that calls no transcendental.  (cornell_box's
+    golden itself differs from the reference's current build by one step on a
+    few pixels; the oracle reproduces the current build.)"""
     hs = load(name)
     sid = core.upload(hs.desc())
     try:
@@ -256,9 +265,9 @@ def test_render_matches_reference_golden_png(core, name):
     rgb = pa.tonemap_rgb8(img)
     ref = np.asarray(Image.open(golden(name + ".png")).convert("RGB"))
     d = np.abs(rgb.astype(np.int16) - ref.astype(np.int16))
-    within = (d <= 1).all(-1).mean()
-    print(f"{name}: max={d.max()} within1={within:.6f} exact={(d == 0).all(-1).mean():.6f}")
-    assert within >= 0.999
+    parity_log(f"golden {name}.png", max8=int(d.max()), exact8=float((d == 0).all(-1).mean()),
+               within1=float((d <= 1).all(-1).mean()))
+    assert d.max() <= 1
     if TRIG_FREE[name]:
         assert d.max() == 0
 
@@ -321,8 +330,14 @@ def test_depth_zero_and_one_match_wavefront_and_oracle(core, name, depth):
     assert np.array_equal(mk.view(np.uint32), wf.view(np.uint32))   # bit for bit, signed zeros included
     assert (mst.rays_traced, mst.closest_queries, mst.anyhit_queries) == \
         (wst.rays_traced, wst.closest_queries, wst.anyhit_queries)
-    oimg, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    osc = ol.OracleScene(hs.desc())
+    oimg, oc = osc.render(hs.camera(), s)
     assert (mst.rays_traced, mst.closest_queries, mst.anyhit_queries) == \
         (oc.rays_traced, oc.closest_queries, oc.anyhit_queries)
-    exact = np.all(mk == oimg, axis=-1).mean()
-    assert exact == 1.0 if TRIG_FREE[name] else exact >= 0.99
+    f = img_stats(mk, oimg)
+    if TRIG_FREE[name]:
+        assert f["exact"] == 1.0
+    assert_reference_bar(f"{name} depth {depth}", f)
+    with ol.trig_mode(1):
+        cimg, _ = osc.render(hs.camera(), s)
+    assert np.array_equal(mk, cimg)

@@ -135,6 +135,53 @@ def test_oracle_stress_full_topology_matches_reference():
     assert np.array_equal(hits["t"], fx["full_hit_t"])
 
 
+def test_oracle_cornell_pt_bands_match_reference():
+    """The path-traced workload at full size (1920x1080, 256 spp, 8 bounces,
+    Russian roulette, 1,048,820-triangle mesh): three row bands of the
+    reference's own render, bit for bit, with the reference's ray counts."""
+    from pencil_amd import workloads as wl
+    fx = np.load(golden("cfg_cornell_pt.npz"))
+    hs, st = wl.cornell_pt(pa)
+    assert str(fx["opts"]).split()[1::2][:3] == [str(st.width), str(st.height), str(st.samples)]
+    osc = ol.OracleScene(hs.desc())
+    for k, (r0, r1) in enumerate(fx["bands"]):
+        img, c = osc.render(hs.camera(), st, rows=(int(r0), int(r1)))
+        assert np.array_equal(img[r0:r1], fx["hdr"][k])
+        assert c.rays_traced == int(fx["rays"][k])
+
+
+def test_oracle_batch64_bands_match_reference():
+    """Three scenes of the 64-scene batched workload (meshes, metal / glass
+    spheres, a soft light, 16 spp, Whitted depth 6) on their metal- and
+    glass-rich row bands: the reference's output bit for bit."""
+    from pencil_amd import workloads as wl
+    fx = np.load(golden("cfg_batch64.npz"))
+    specs = json.loads(str(fx["specs"]))
+    assert [wl.batch64_specs()[int(k)] for k in fx["picks"]] == specs
+    for k, sp in enumerate(specs):
+        hs, st = wl.batch64_scene(pa, sp)
+        r0, r1 = (int(x) for x in fx[f"band{k}"])
+        img, c = ol.OracleScene(hs.desc()).render(hs.camera(), st, rows=(r0, r1))
+        assert np.array_equal(img[r0:r1], fx[f"hdr{k}"])
+        assert c.rays_traced == int(fx[f"rays{k}"])
+
+
+def test_oracle_trig_mode_changes_only_rounding():
+    """trig_mode(1) (cos / sin / pow evaluated in double, rounded once) moves
+    a soft-shadow scene by rounding only: most pixels identical, none by an
+    8-bit step; mode 0 is restored on exit."""
+    hs = load("soft_shadows", width=64, height=48, samples=2)
+    osc = ol.OracleScene(hs.desc())
+    a, _ = osc.render(hs.camera(), hs.settings())
+    with ol.trig_mode(1):
+        b, _ = osc.render(hs.camera(), hs.settings())
+    c, _ = osc.render(hs.camera(), hs.settings())
+    assert np.array_equal(a, c)
+    assert not np.array_equal(a, b)
+    assert np.all(a == b, axis=-1).mean() >= 0.99
+    assert np.array_equal(ol.tonemap_rgb8(a), ol.tonemap_rgb8(b))
+
+
 def test_xorshift32_known_answers():
     # Marsaglia (2003) xorshift32 with shifts 13, 17, 5, worked by hand:
     # x=1: x^=x<<13 -> 8193; x^=x>>17 -> 8193; x^=x<<5 -> 8193^262176 = 270369
