@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 4
+#define PCL_ABI_VERSION 5
 
 enum {
     PCL_OK              = 0,
@@ -328,6 +328,23 @@ int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_se
 /* Same with the target (and the optional image output) in device memory. */
 int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                        const float* d_target, double* grad, double* loss, float* d_out_rgb, pcl_stats* stats);
+
+/* pcl_adjoint_device without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during the
+ * call (code under HIP graph capture, tight optimisation loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results (bit for bit)
+ * as pcl_adjoint_device; its work buffers (per-8x8-tile FP64 partials, the
+ * reduced gradient, counters) live in a fixed array of
+ * PCL_NOALLOC_SCRATCH_BYTES in device memory, one per GPU, held until the
+ * process ends.  Launches needing more than that -- 8x8 tiles x (6 x
+ * materials + 1) doubles, e.g. about 2.7 M pixels with 8 materials -- fail
+ * with PCL_ERR_LIMIT before anything is enqueued.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other threads,
+ * other contexts) race and must be serialised by the caller. */
+#define PCL_NOALLOC_SCRATCH_BYTES (16u << 20)
+int pcl_adjoint_device_noalloc(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                               const float* d_target, double* grad, double* loss, float* d_out_rgb,
+                               pcl_stats* stats);
 
 /* Device-memory helpers for callers without their own allocator. */
 int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr);

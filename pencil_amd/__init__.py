@@ -152,6 +152,8 @@ def hip_lib():
              [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, _P, _P, C.POINTER(Stats)])
         _sig(lib, "pcl_adjoint_device", C.c_int,
              [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, _P, _P, C.POINTER(Stats)])
+        _sig(lib, "pcl_adjoint_device_noalloc", C.c_int,
+             [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, _P, _P, _P, C.POINTER(Stats)])
         _sig(lib, "pcl_device_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)])
         _sig(lib, "pcl_device_free", C.c_int, [_P, _P])
         _sig(lib, "pcl_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t])
@@ -406,6 +408,19 @@ class Core:
         self._check(self.lib.pcl_adjoint_device(self.ctx, sid, C.byref(cam), C.byref(settings), C.c_void_p(d_target),
                                                 _ptr(grad), C.byref(loss), C.c_void_p(d_out or None), C.byref(st)),
                     "pcl_adjoint_device")
+        return grad[:n], loss.value, st
+
+    def adjoint_device_noalloc(self, sid: int, cam: Camera, settings: Settings, d_target: int, d_out: int = 0):
+        """adjoint_device through pcl_adjoint_device_noalloc (work buffers in the
+        library's fixed per-device scratch; nothing allocated)."""
+        n = self._nmat[sid]
+        grad = np.zeros((max(n, 1), 6), dtype=np.float64)
+        loss = C.c_double()
+        st = Stats()
+        self._check(self.lib.pcl_adjoint_device_noalloc(self.ctx, sid, C.byref(cam), C.byref(settings),
+                                                        C.c_void_p(d_target), _ptr(grad), C.byref(loss),
+                                                        C.c_void_p(d_out or None), C.byref(st)),
+                    "pcl_adjoint_device_noalloc")
         return grad[:n], loss.value, st
 
     def render_vjp_device(self, sid: int, cam: Camera, settings: Settings, d_adj: int):

@@ -121,6 +121,23 @@ __global__ __launch_bounds__(kBlock, PT == 1 ? 4 : 3) void k_render(DevScene S, 
     flush_counts(c, ctr);
 }
 
+// Scratch memory of pcl_adjoint_device_noalloc, one copy per device (held
+// until the process ends).  Layout: 16 counter words at offset 0, the
+// reduced gradient + loss (PCL_ADJ_MAX_MATERIALS * 6 + 1 doubles) at
+// kScrOut, the per-tile partials from kScrPartial on.  Host code never uses
+// this name as an address: pcl_init looks up each device's copy with
+// hipGetSymbolAddress and keeps it in the context.
+constexpr size_t kScratchBytes = (size_t)PCL_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScrOut = 256;
+constexpr size_t kScrPartial = 4096;
+static_assert(kScrOut + (PCL_ADJ_MAX_MATERIALS * 6 + 1) * sizeof(double) <= kScrPartial, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 struct SceneSlot {
     bool used = false;
     DevScene dev{};
@@ -153,6 +170,7 @@ struct pcl_ctx {
     float* d_thr = nullptr;          // tone-map threshold table (256 floats)
     uint8_t* d_rgb8 = nullptr;
     size_t rgb8_cap = 0;
+    unsigned char* d_scratch = nullptr;   // this device's copy of `scratch`
 };
 
 namespace {
@@ -255,6 +273,9 @@ int pcl_init(int device, pcl_ctx** out) {
         if (hipMalloc(&ctx->d_thr, sizeof(thr)) != hipSuccess) return PCL_ERR_OOM;
         if (hipMemcpy(ctx->d_thr, thr, sizeof(thr), hipMemcpyHostToDevice) != hipSuccess) return PCL_ERR_HIP;
     }
+    // the device address of this device's scratch array (device is current)
+    if (hipGetSymbolAddress((void**)&ctx->d_scratch, HIP_SYMBOL(scratch)) != hipSuccess || !ctx->d_scratch)
+        return PCL_ERR_HIP;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cus = prop.multiProcessorCount;
@@ -894,6 +915,53 @@ int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     const double t0 = now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     return adj_run(ctx, sl, cam, settings, d_target, nullptr, d_out_rgb, grad, loss, stats, t0);
+}
+
+// pcl_adjoint_device with every work buffer in the device's scratch array:
+// no allocation, the same kernels.  The counters are cleared with a kernel
+// (a capturable node); partials and result are written in full by the pass.
+int pcl_adjoint_device_noalloc(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
+                               const float* d_target, double* grad, double* loss, float* d_out_rgb,
+                               pcl_stats* stats) {
+    SceneSlot* sl = get_scene(ctx, scene);
+    int rc = adj_check(ctx, sl, cam, settings, grad);
+    if (rc) return rc;
+    if (!d_target) return fail(ctx, PCL_ERR_INVALID, "null target");
+    if (!ctx->d_scratch) return fail(ctx, PCL_ERR_HIP, "no scratch memory on this device");
+    const double t0 = now_ms();
+    const RSettings st = to_dev(settings);
+    const size_t need = pcl::adj::partial_doubles(st, sl->n_mats) * sizeof(double);
+    if (need > kScratchBytes - kScrPartial)
+        return fail(ctx, PCL_ERR_LIMIT, "adjoint without allocation: " + std::to_string(need) +
+                                            " bytes of per-tile partials, the scratch holds " +
+                                            std::to_string(kScratchBytes - kScrPartial) +
+                                            " (8x8 tiles x (6 x materials + 1) doubles)");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const CamRec C = make_camera(*cam);
+    unsigned long long* d_ctr = reinterpret_cast<unsigned long long*>(ctx->d_scratch);
+    double* d_out = reinterpret_cast<double*>(ctx->d_scratch + kScrOut);
+    double* d_part = reinterpret_cast<double*>(ctx->d_scratch + kScrPartial);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, d_ctr, kCtrSlots, 0ull);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, pcl::adj::launch(ctx->stream, sl->dev, C, st, sl->n_mats, d_target, nullptr, d_out_rgb, d_part,
+                                  d_out, d_ctr));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    double hg[PCL_ADJ_MAX_MATERIALS * 6 + 1];
+    unsigned long long h[kCtrSlots];
+    const size_t n_par = (size_t)sl->n_mats * 6;
+    HIP_TRY(ctx, hipMemcpyAsync(hg, d_out, (n_par + 1) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(h, d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    for (size_t p = 0; p < n_par; ++p) grad[p] = hg[p];
+    if (loss) *loss = hg[n_par];
+    if (stats) {
+        *stats = pcl_stats{};
+        fill_stats(stats, h, ms, t0);
+    }
+    return PCL_OK;
 }
 
 int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
