@@ -20,10 +20,18 @@ node entered (four child boxes + four refs) + the primitive's bytes per test (sp
 plane 32, cylinder 48, cone 48), counted by an instrumented launch of the
 same frame; divided by the kernel's mean HIP-event duration.
 
-cpu_baseline: the reference renderer (oracle/_ref/pencil, built from the
-reference sources) on the same scene written as .pecl + .obj, all host cores
-used (capped at 16), render() time as the reference prints it; falls back to
-the oracle port if the reference binary was not built.
+cpu_baseline: the reference renderer itself -- its parser, BVH and
+Renderer::render, driven by oracle/_ref/ref_harness (built from the reference
+sources by oracle/Makefile) on the same scene written as .pecl + .obj -- with
+every host core this process may run on (os.sched_getaffinity), render()
+timed alone as src/pencil.cpp:311-313 does (parse and BVH build untimed),
+the median of 5 renders after one warm-up in the same process.  The CPU
+model and core count are reported with it.  Without the reference build the
+baseline is reported as missing (never replaced by the oracle).
+
+Every leg carries its own "roofline" (algorithmic bytes of its queries from
+an instrumented launch, over the leg's kernel time) and "cpu_baseline"
+(the reference on a bounded sample of the same workload).
 
 Further workloads ("legs", each reported under "legs" in the JSON line):
   cornell_pt   Cornell box (pencil_amd/scenes/cornell_c5.pecl) with the
@@ -44,8 +52,8 @@ Further workloads ("legs", each reported under "legs" in the JSON line):
 import argparse
 import json
 import os
-import re
 import shutil
+import statistics
 import subprocess
 import sys
 import tempfile
@@ -56,12 +64,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from pencil_amd import workloads as wl  # noqa: E402
+
 PRIM_BYTES = [16, 48, 16, 32, 48, 48]   # sphere, triangle, cube, plane, cylinder, cone
 NODE_BYTES = 112                         # a 4-wide node read: 6 x 16 B of child boxes + 16 B of refs
 HBM_PEAK_GBS = 8000.0
-STRESS = dict(freq=229, n_spheres=4096, seed=1234, width=1920, height=1080)
-CORNELL = os.path.join(ROOT, "pencil_amd", "scenes", "cornell_c5.pecl")
-MESH_IN_BOX = dict(freq=229, seed=1234, scale=0.25, center=(0.0, 1.75, 0.5))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(*a):
@@ -79,8 +88,19 @@ def cfg_str(c):
             f"drain below {c.drain_below}, inline shadows {c.inline_shadows}")
 
 
-def cpu_threads():
-    return min(16, os.cpu_count() or 1)
+def host_cpu():
+    """(threads this process may use, CPU model)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
 
 
 def oracle():
@@ -89,50 +109,83 @@ def oracle():
     return oracle_lib
 
 
-def cpu_baseline(queries, threads):
-    """Reference renderer on the bvh_stress frame; returns dict or None."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "pencil")
-    import pencil_amd as pa
-    tmp = tempfile.mkdtemp(prefix="pcl_bench_")
-    try:
-        if os.path.exists(ref):
-            pa.HostScene.write_stress(tmp, "stress", **STRESS)
-            out = os.path.join(tmp, "out.ppm")
-            t0 = time.time()
-            r = subprocess.run([ref, os.path.join(tmp, "stress.pecl"), out, "--threads", str(threads)],
-                               capture_output=True, text=True, timeout=600)
-            wall = time.time() - t0
-            m = re.search(r"rendered in ([0-9.]+) s \(([0-9,]+) rays", r.stdout)
-            if r.returncode != 0 or not m:
-                log("reference run failed:", r.returncode, r.stderr[-500:])
-                return None
-            sec = float(m.group(1))
-            return {"value": round(queries / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads,
-                    "kind": "reference",
-                    "sample": f"full 1920x1080 frame, reference binary built from the reference sources, "
-                              f"render() {sec:.3f} s as it prints it ({m.group(2)} reference-convention rays; "
-                              f"{wall:.1f} s wall incl. .pecl/.obj parse and BVH build)"}
-        ol = oracle()
-        hs = pa.HostScene.stress(**STRESS)
-        osc = ol.OracleScene(hs.desc())
-        s = hs.settings()
-        rows = (0, 270)   # a quarter of the frame
-        t0 = time.time()
-        _, c = osc.render(hs.camera(), s, threads=threads, rows=rows)
-        sec = time.time() - t0
-        q = c.closest_queries + c.anyhit_queries
-        return {"value": round(q / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-                "sample": f"rows 0-269 of the 1920x1080 frame through the oracle ({q} queries, {sec:.2f} s)"}
-    finally:
-        shutil.rmtree(tmp, ignore_errors=True)
-
-
 def rmse(a, b):
     return float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
 
 
 def rounded(d):
-    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}
+    if isinstance(d, dict):
+        return {k: rounded(v) for k, v in d.items()}
+    if isinstance(d, float):
+        return round(d, 4) if abs(d) >= 1e-3 or d == 0 else float(f"{d:.4g}")
+    return d
+
+
+def alg_bytes(st, wc):
+    """Algorithmic bytes of a launch's queries: 32 B ray + 8 B hit per query, 112 B per
+    4-wide node entered, the primitive's bytes per test."""
+    q = st.closest_queries + st.anyhit_queries
+    return q * (32 + 8) + NODE_BYTES * wc.node_visits + sum(b * wc.prim_tests[k] for k, b in enumerate(PRIM_BYTES))
+
+
+def profile_traffic(name):
+    """HBM bytes per launch (or per job) from a committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(PROFILES, name + ".traffic.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    return int(round(t["hbm_bytes_per_dispatch"])), t
+
+
+def roofline(bytes_per_launch, kernel_ms, traffic_name, scope):
+    """Algorithmic-byte roofline of a kernel (or a job of kernels), with the
+    counter-measured HBM traffic beside it."""
+    achieved = bytes_per_launch / (kernel_ms / 1000.0) / 1e9
+    traffic, t = profile_traffic(traffic_name)
+    r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "scope": scope,
+         "algorithmic_bytes": int(bytes_per_launch)}
+    if traffic:
+        # the rate the counters see, over the kernel time measured here
+        r["measured_hbm_gbs"] = round(traffic / (kernel_ms / 1000.0) / 1e9, 1)
+        r["measured_frac"] = round(traffic / (kernel_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+        r["traffic_profile"] = f"profiles/{traffic_name}.md"
+    return r
+
+
+# ---------------------------------------------------------------------------
+# the reference renderer on the host cores (CPU baseline)
+# ---------------------------------------------------------------------------
+
+def ref_render(pecl, opts, threads, rows=None, repeat=6):
+    """Renders with the reference (parse and BVH build once, untimed), `repeat`
+    times; returns (median of the renders after the first, all render seconds,
+    reference-convention rays, HDR image or band)."""
+    out = pecl + ".bin"
+    cmd = [HARNESS, "render", pecl, out] + opts + ["--threads", str(threads), "--repeat", str(repeat), "--no-tag"]
+    if rows:
+        cmd += ["--rows", str(rows[0]), str(rows[1])]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(f"reference render failed ({r.returncode}): {r.stderr[-500:]}")
+    secs, rays = None, None
+    for line in r.stdout.splitlines():
+        if line.startswith("seconds"):
+            secs = [float(x) for x in line.split()[1:]]
+        elif line.startswith("rays"):
+            rays = int(line.split()[1])
+    raw = open(out, "rb").read()
+    W, H = np.frombuffer(raw[:8], dtype=np.int32)
+    img = np.frombuffer(raw[8:8 + W * H * 12], dtype=np.float32).reshape(H, W, 3).copy()
+    timed = secs[1:] if len(secs) > 1 else secs
+    return statistics.median(timed), secs, rays, img
+
+
+def cpu_entry(value, unit, threads, model, sample, secs):
+    return {"value": round(value, 6) if value < 1 else round(value, 3), "unit": unit, "cores": threads,
+            "kind": "reference", "cpu_model": model, "sample": sample,
+            "render_s_runs": [round(x, 4) for x in secs]}
 
 
 # ---------------------------------------------------------------------------
@@ -141,8 +194,8 @@ def rounded(d):
 
 def leg_stress(core, pa, args):
     t0 = time.time()
-    hs = pa.HostScene.stress(**STRESS)
-    desc, cam, st = hs.desc(), hs.camera(), hs.settings()
+    hs, st = wl.stress(pa)
+    desc, cam = hs.desc(), hs.camera()
     log(f"scene: {desc.n_prims} primitives, generated in {time.time() - t0:.2f} s")
     sid = core.upload(desc)
     info = core.scene_info(sid)
@@ -154,10 +207,10 @@ def leg_stress(core, pa, args):
     # instrumented frame: work counts for the bytes model (not timed)
     cst, wc = core.render_counts(sid, cam, st, d_out)
     queries = cst.closest_queries + cst.anyhit_queries
-    tests = [wc.prim_tests[k] for k in range(6)]
-    bytes_per_launch = queries * (32 + 8) + NODE_BYTES * wc.node_visits + sum(b * t for b, t in zip(PRIM_BYTES, tests))
+    bytes_per_launch = alg_bytes(cst, wc)
     log(f"counts: {cst.closest_queries} closest + {cst.anyhit_queries} any-hit queries, "
-        f"{wc.node_visits} node visits, tests {tests}, {bytes_per_launch / 1e9:.2f} GB algorithmic per frame")
+        f"{wc.node_visits} node visits, tests {[wc.prim_tests[k] for k in range(6)]}, "
+        f"{bytes_per_launch / 1e9:.2f} GB algorithmic per frame")
 
     for _ in range(args.warmup):
         core.render_device(sid, cam, st, d_out)
@@ -170,7 +223,6 @@ def leg_stress(core, pa, args):
     ms_per_step = (t_end - t_start) * 1000.0 / args.steps
     kernel_ms = sum(kms) / len(kms)
     value = queries / (ms_per_step / 1000.0) / 1e6
-    achieved = bytes_per_launch / (kernel_ms / 1000.0) / 1e9
     log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame wall, {kernel_ms:.3f} ms kernel (HIP events)")
 
     batched = None
@@ -197,10 +249,13 @@ def leg_stress(core, pa, args):
         core.d2h(b, d_wf)
         wq = bs.closest_queries + bs.anyhit_queries
         wms = (t_end - t_start) * 1000.0 / args.steps
-        batched = {"ms_per_step": wms, "kernel_ms": sum(wkms) / len(wkms), "queries": wq,
-                   "mrays": wq / (wms / 1000.0) / 1e6, "rmse_vs_plain": rmse(a, b), "iterations": bs.iterations,
-                   "config": cfg_dict(bs.used)}
-        log(f"batched: {wms:.3f} ms/frame wall, {batched['kernel_ms']:.3f} ms on the stream, "
+        wk = sum(wkms) / len(wkms)
+        batched = {"ms_per_step": wms, "kernel_ms": wk, "queries": wq,
+                   "mrays": wq / (wms / 1000.0) / 1e6, "rmse_vs_plain": rmse(a, b),
+                   "bit_identical_to_plain": bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))),
+                   "iterations": bs.iterations, "config": cfg_dict(bs.used),
+                   "roofline": roofline(bytes_per_launch, wk, "wavefront_bvh_stress_1080p", "job")}
+        log(f"batched: {wms:.3f} ms/frame wall, {wk:.3f} ms on the stream, "
             f"{batched['mrays']:.1f} Mrays/s, RMSE vs plain {batched['rmse_vs_plain']:.3g}")
         core.free(d_wf)
     # the device BVH builder (use_bvh = 2) on the same scene: build time and frame time
@@ -224,29 +279,48 @@ def leg_stress(core, pa, args):
     core.free_scene(sid2)
     core.free(d_out)
     core.free_scene(sid)
-    return dict(value=value, ms_per_step=ms_per_step, kernel_ms=kernel_ms, achieved=achieved, queries=queries,
+    return dict(value=value, ms_per_step=ms_per_step, kernel_ms=kernel_ms, bytes=bytes_per_launch, queries=queries,
                 cst=cst, info=info, desc_n=desc.n_prims, W=W, H=H, batched=batched, lbvh=lbvh)
+
+
+def cpu_stress(queries, threads, model):
+    """The reference on the whole headline frame."""
+    tmp = tempfile.mkdtemp(prefix="pcl_bench_")
+    try:
+        import pencil_amd as pa
+        pa.HostScene.write_stress(tmp, "stress", **wl.STRESS)
+        t0 = time.time()
+        med, secs, rays, _ = ref_render(os.path.join(tmp, "stress.pecl"), [], threads)
+        wall = time.time() - t0
+        return cpu_entry(queries / med / 1e6, "Mrays/s", threads, model,
+                         f"the full 1920x1080 frame: Renderer::render of the reference build, median of 5 renders "
+                         f"after a warm-up ({med:.3f} s; {rays:,} reference-convention rays; the {queries:,} "
+                         f"queries counted by the core on the same frame; {wall:.1f} s wall incl. .pecl/.obj "
+                         f"parse and BVH build)", secs)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
 
 
 # ---------------------------------------------------------------------------
 # cornell_pt: path tracing, plain vs batched
 # ---------------------------------------------------------------------------
 
-def leg_cornell_pt(core, pa, args, threads):
-    spp_side = args.pt_samples
-    hs = pa.HostScene.cornell_mesh(CORNELL, width=1920, height=1080, **MESH_IN_BOX)
-    desc, cam, st = hs.desc(), hs.camera(), hs.settings()
-    st.samples, st.pathtrace, st.pathtrace_bounces = spp_side, 1, 8
+def leg_cornell_pt(core, pa, args, threads, model):
+    hs, st = wl.cornell_pt(pa, samples=args.pt_samples)
+    desc, cam = hs.desc(), hs.camera()
     sid = core.upload(desc)
     W, H = st.width, st.height
     d_a = core.alloc(W * H * 12)
     d_b = core.alloc(W * H * 12)
     try:
+        cst, wc = core.render_counts(sid, cam, st, d_b)   # instrumented: the bytes model's units
+        nbytes = alg_bytes(cst, wc)
         t0 = time.perf_counter()
         ps = core.render_device(sid, cam, st, d_a)
         plain_ms = (time.perf_counter() - t0) * 1000.0
         q = ps.closest_queries + ps.anyhit_queries
-        log(f"cornell_pt plain: {plain_ms:.1f} ms, {q} queries, {q / plain_ms / 1e3:.1f} Mrays/s")
+        log(f"cornell_pt plain: {plain_ms:.1f} ms, {q} queries, {q / plain_ms / 1e3:.1f} Mrays/s, "
+            f"{nbytes / 1e9:.1f} GB algorithmic")
         if args.batched_config:
             cfg = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")])
         else:
@@ -269,26 +343,39 @@ def leg_cornell_pt(core, pa, args, threads):
         core.free(d_a)
         core.free(d_b)
         core.free_scene(sid)
-    out = {"workload": f"Cornell box + {desc.n_prims - 14}-tri mesh, {W}x{H}, {spp_side ** 2} spp, path tracing, "
+    out = {"workload": f"Cornell box + {desc.n_prims - 14}-tri mesh, {W}x{H}, {st.samples ** 2} spp, path tracing, "
                        f"8 bounces, Russian roulette after 3",
            "primitives": desc.n_prims, "queries": q, "ref_convention_rays": ps.rays_traced,
-           "plain_ms": plain_ms, "plain_mrays_s": q / plain_ms / 1e3,
+           "plain_ms": plain_ms, "plain_kernel_ms": ps.kernel_ms, "plain_mrays_s": q / plain_ms / 1e3,
            "batched_ms": wf_ms, "batched_mrays_s": wq / wf_ms / 1e3, "batched_kernel_ms": bs.kernel_ms,
            "batched_iterations": bs.iterations, "batched_config": cfg_dict(bs.used),
            "batched_pixels_finished_after_drain": bs.finished_pixels,
            "speedup_batched_vs_plain": plain_ms / wf_ms, "rmse_batched_vs_plain": rmse(a, b),
-           "image_mean": float(a.mean())}
-    if not args.no_cpu:
-        ol = oracle()
-        osc = ol.OracleScene(desc)
+           "bit_identical": bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))),
+           "image_mean": float(a.mean()),
+           "roofline": roofline(nbytes, ps.kernel_ms, "k_render_cornell_pt_1080p_256spp", "kernel"),
+           "batched_roofline": roofline(nbytes, bs.kernel_ms, "wavefront_cornell_pt_1080p_256spp", "job")}
+    if not args.no_cpu and os.path.exists(HARNESS):
         rows = (H // 2 - 2, H // 2 + 2)
-        t0 = time.time()
-        _, c = osc.render(cam, st, threads=threads, rows=rows)
-        sec = time.time() - t0
+        tmp = tempfile.mkdtemp(prefix="pcl_bench_")
+        try:
+            pa.HostScene.write_cornell_mesh(tmp, "cornell_pt", wl.CORNELL, **wl.MESH_IN_BOX)
+            opts = ["--width", str(W), "--height", str(H), "--samples", str(st.samples), "--pathtrace", "--bounces",
+                    str(st.pathtrace_bounces)]
+            med, secs, rays, band = ref_render(os.path.join(tmp, "cornell_pt.pecl"), opts, threads, rows=rows)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        # the oracle checks the sample (bit for bit: same pixels, same rays) and counts its queries
+        ol = oracle()
+        img, c = ol.OracleScene(desc).render(cam, st, threads=threads, rows=rows)
+        same = bool(np.array_equal(img[rows[0]:rows[1]], band)) and c.rays_traced == rays
         cq = c.closest_queries + c.anyhit_queries
-        out["cpu_baseline"] = {"value": round(cq / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-                               "sample": f"rows {rows[0]}-{rows[1] - 1} (the middle of the frame) at 256 spp "
-                                         f"through the oracle ({cq} queries, {sec:.2f} s)"}
+        out["cpu_baseline"] = cpu_entry(
+            cq / med / 1e6, "Mrays/s", threads, model,
+            f"rows {rows[0]}-{rows[1] - 1} (the middle of the frame) at {st.samples ** 2} spp: render_tile_region of "
+            f"the reference build, median of 5 after a warm-up ({med:.3f} s; {rays:,} reference-convention rays, "
+            f"{cq:,} queries as the oracle counts them on the same rows, its pixels "
+            f"{'identical' if same else 'NOT identical'} to the reference's)", secs)
     return rounded(out)
 
 
@@ -296,33 +383,21 @@ def leg_cornell_pt(core, pa, args, threads):
 # batch64: 64 uneven scenes in one batched job vs 64 plain renders
 # ---------------------------------------------------------------------------
 
-def batch64_specs(n=64, seed=2024):
-    rng = np.random.default_rng(seed)
-    specs = []
-    for k in range(n):
-        tris = float(np.exp(rng.uniform(np.log(16), np.log(500_000))))
-        freq = max(1, int(round(np.sqrt(tris / 20.0))))
-        res = int(rng.choice([256, 512, 1024]))
-        specs.append(dict(freq=freq, n_spheres=int(rng.integers(8, 257)), seed=int(seed * 1000 + k),
-                          width=res, height=res))
-    return specs
-
-
-def leg_batch64(core, pa, args, threads):
-    specs = batch64_specs()
+def leg_batch64(core, pa, args, threads, model):
+    specs = wl.batch64_specs()
     t0 = time.time()
-    scenes = []
-    for sp in specs:
-        hs = pa.HostScene.mixed(**sp)
-        st = hs.settings()
-        st.samples, st.max_depth, st.pathtrace = 4, 6, 0
-        scenes.append((hs, st))
+    scenes = wl.batch64(pa)
     sids = [core.upload(hs.desc()) for hs, _ in scenes]
     n_prims = sum(hs.desc().n_prims for hs, _ in scenes)
     log(f"batch64: {len(scenes)} scenes, {n_prims} primitives, built and uploaded in {time.time() - t0:.1f} s")
     outs = [core.alloc(st.width * st.height * 12) for _, st in scenes]
     outs2 = [core.alloc(st.width * st.height * 12) for _, st in scenes]
+    per_scene_q = []
     try:
+        nbytes = 0
+        for sid, (hs, st), d in zip(sids, scenes, outs2):
+            cst, wc = core.render_counts(sid, hs.camera(), st, d)
+            nbytes += alg_bytes(cst, wc)
         job = [(sid, hs.camera(), st, d) for sid, (hs, st), d in zip(sids, scenes, outs)]
         if args.batched_config:
             cfg = pa.WfConfig(*[int(x) for x in args.batched_config.split(",")])
@@ -343,18 +418,21 @@ def leg_batch64(core, pa, args, threads):
         for sid, (hs, st), d in zip(sids, scenes, outs2):   # warm-up
             core.render_device(sid, hs.camera(), st, d)
         t0 = time.perf_counter()
-        pq = 0
+        pq, pk = 0, 0.0
         for sid, (hs, st), d in zip(sids, scenes, outs2):
             s = core.render_device(sid, hs.camera(), st, d)
             pq += s.closest_queries + s.anyhit_queries
+            pk += s.kernel_ms
+            per_scene_q.append(s.closest_queries + s.anyhit_queries)
         plain_ms = (time.perf_counter() - t0) * 1000.0
-        err = 0.0
+        err, same = 0.0, True
         for (hs, st), d, d2 in zip(scenes, outs, outs2):
             a = np.empty((st.height, st.width, 3), np.float32)
             b = np.empty_like(a)
             core.d2h(a, d)
             core.d2h(b, d2)
             err = max(err, rmse(a, b))
+            same = same and bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
     finally:
         for d in outs + outs2:
             core.free(d)
@@ -368,21 +446,33 @@ def leg_batch64(core, pa, args, threads):
            "queries": wq, "batched_ms": wf_ms, "batched_mrays_s": wq / wf_ms / 1e3,
            "batched_kernel_ms": bs.kernel_ms, "batched_iterations": bs.iterations,
            "batched_config": cfg_dict(bs.used), "batched_pixels_finished_after_drain": bs.finished_pixels,
-           "plain_sequential_ms": plain_ms, "plain_sequential_mrays_s": pq / plain_ms / 1e3,
-           "max_rmse_batched_vs_plain": err}
-    if not args.no_cpu:
-        ol = oracle()
-        small = [i for i, (_, st) in enumerate(scenes) if st.width == 256][:2]
-        t0 = time.time()
-        cq = 0
-        for i in small:
-            hs, st = scenes[i]
-            _, c = ol.OracleScene(hs.desc()).render(hs.camera(), st, threads=threads)
-            cq += c.closest_queries + c.anyhit_queries
-        sec = time.time() - t0
-        out["cpu_baseline"] = {"value": round(cq / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-                               "sample": f"scenes {small} (256^2) rendered whole through the oracle, incl. its BVH "
-                                         f"build ({cq} queries, {sec:.2f} s)"}
+           "plain_sequential_ms": plain_ms, "plain_sequential_kernel_ms": pk,
+           "plain_sequential_mrays_s": pq / plain_ms / 1e3, "speedup_batched_vs_plain": plain_ms / wf_ms,
+           "max_rmse_batched_vs_plain": err, "bit_identical": same,
+           "roofline": roofline(nbytes, bs.kernel_ms, "wavefront_batch64_16spp", "job"),
+           "plain_roofline": roofline(nbytes, pk, "k_render_batch64_16spp", "64 kernels")}
+    if not args.no_cpu and os.path.exists(HARNESS):
+        # a bounded sample: the job's first 4 scenes, each rendered whole by the reference
+        sample = list(range(4))
+        tmp = tempfile.mkdtemp(prefix="pcl_bench_")
+        meds, all_secs = [], []
+        try:
+            for k in sample:
+                pa.HostScene.write_stress(tmp, f"b{k}", mat_mix=1, **specs[k])
+                med, secs, _, _ = ref_render(os.path.join(tmp, f"b{k}.pecl"),
+                                             ["--samples", str(wl.BATCH64["samples"]), "--depth",
+                                              str(wl.BATCH64["depth"])], threads)
+                meds.append(med)
+                all_secs.append(secs)
+        finally:
+            shutil.rmtree(tmp, ignore_errors=True)
+        cq = sum(per_scene_q[k] for k in sample)
+        desc_s = ", ".join(f"{specs[k]['width']}^2/{20 * specs[k]['freq'] ** 2} tris" for k in sample)
+        out["cpu_baseline"] = cpu_entry(
+            cq / sum(meds) / 1e6, "Mrays/s", threads, model,
+            f"scenes {sample} of the job ({desc_s}) rendered whole, one after another, by Renderer::render of the "
+            f"reference build: sum of per-scene medians of 5 after a warm-up ({sum(meds):.3f} s; {cq:,} queries "
+            f"as the core counts them on the same scenes)", [round(sum(x), 4) for x in zip(*all_secs)])
     return rounded(out)
 
 
@@ -390,17 +480,18 @@ def leg_batch64(core, pa, args, threads):
 # inverse: adjoint gradient steps
 # ---------------------------------------------------------------------------
 
-def leg_inverse(core, pa, args, threads, steps=200):
-    size, spp_side, bounces = 512, 8, 4
-    hs = pa.HostScene.from_pecl(CORNELL, size, size, spp_side, 4)
-    desc, cam, st = hs.desc(), hs.camera(), hs.settings()
-    st.pathtrace, st.pathtrace_bounces = 1, bounces
+def leg_inverse(core, pa, args, threads, model, steps=200):
+    hs, st = wl.inverse(pa)
+    desc, cam = hs.desc(), hs.camera()
+    size, spp_side, bounces = st.width, st.samples, st.pathtrace_bounces
     sid = core.upload(desc)
     mats, n = pa.materials_of(desc)
     true = pa.material_params(mats, n)
     n_px = size * size * 3
     d_tgt = core.alloc(n_px * 4)
     try:
+        cst, wc = core.render_counts(sid, cam, st, d_tgt)    # queries of one pass (the adjoint replays them)
+        nbytes = alg_bytes(cst, wc)
         core.render_device(sid, cam, st, d_tgt)   # target from the true parameters
         x = np.random.default_rng(7).uniform(0.2, 0.8, size=true.shape)
         pa.set_material_params(mats, n, x)
@@ -454,16 +545,18 @@ def leg_inverse(core, pa, args, threads, steps=200):
         core.free(d_tgt)
         core.free_scene(sid)
     ms_step = total * 1000.0 / steps
+    gpu_ms = float(np.mean(step_ms))
     log(f"inverse: {steps} Adam steps, {ms_step:.2f} ms/step ({steps / total:.2f} steps/s), "
         f"loss {losses[0]:.4g} -> {losses[-1]:.4g}")
     out = {"workload": f"Cornell box, {size}x{size}, {spp_side ** 2} spp, path tracing, {bounces} bounces, "
                        f"{n} materials x (albedo rgb + emission rgb) = {n * 6} parameters, L2 loss, projected Adam "
                        f"(albedo step 0.02 in [0, 1], emission step 0.1, >= 0)",
            "steps": steps, "grad_steps_per_s": steps / total, "ms_per_step": ms_step,
-           "gpu_ms_per_step": float(np.mean(step_ms)),
+           "gpu_ms_per_step": gpu_ms,
            "loss_first": losses[0], "loss_last": losses[-1], "loss_at_100": losses[min(99, steps - 1)],
            "param_rmse_start_vs_true": None, "fd_check_at_start": fd_err,
-           "fd_check_max_rel_err": max(v["rel_err"] for v in fd_err.values())}
+           "fd_check_max_rel_err": max(v["rel_err"] for v in fd_err.values()),
+           "roofline": roofline(nbytes, gpu_ms, "k_adjoint_cornell_512_64spp", "kernel")}
     x0 = np.random.default_rng(7).uniform(0.2, 0.8, true.shape)
     out["param_rmse_last_vs_true"] = float(np.sqrt(np.mean((x - true) ** 2)))
     out["param_rmse_start_vs_true"] = float(np.sqrt(np.mean((x0 - true) ** 2)))
@@ -471,20 +564,18 @@ def leg_inverse(core, pa, args, threads, steps=200):
     for name, cols in (("albedo", slice(0, 3)), ("emission", slice(3, 6))):
         out[f"{name}_rmse_start_vs_true"] = float(np.sqrt(np.mean((x0[:, cols] - true[:, cols]) ** 2)))
         out[f"{name}_rmse_last_vs_true"] = float(np.sqrt(np.mean((x[:, cols] - true[:, cols]) ** 2)))
-    if not args.no_cpu:
-        ol = oracle()
-        osc = ol.OracleScene(desc)
+    if not args.no_cpu and os.path.exists(HARNESS):
         rows = (size // 2 - 8, size // 2 + 8)
-        t0 = time.time()
-        osc.render(cam, st, threads=threads, rows=rows)
-        sec = time.time() - t0
-        full = sec * size / (rows[1] - rows[0])
+        opts = ["--width", str(size), "--height", str(size), "--samples", str(spp_side), "--pathtrace", "--bounces",
+                str(bounces)]
+        med, secs, rays, _ = ref_render(wl.CORNELL, opts, threads, rows=rows)
+        full = med * size / (rows[1] - rows[0])
         per_grad = full * (2 * n * 6 + 1)
-        out["cpu_baseline"] = {"value": round(1.0 / per_grad, 6), "unit": "grad-steps/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"rows {rows[0]}-{rows[1] - 1} of one 512^2 64-spp forward render through "
-                                         f"the oracle ({sec:.2f} s), scaled to the frame and to the "
-                                         f"{2 * n * 6 + 1} renders of a central-difference gradient"}
+        out["cpu_baseline"] = cpu_entry(
+            1.0 / per_grad, "grad-steps/s", threads, model,
+            f"rows {rows[0]}-{rows[1] - 1} of the {size}^2 {spp_side ** 2}-spp frame: render_tile_region of the "
+            f"reference build, median of 5 after a warm-up ({med:.3f} s), scaled to the frame ({full:.2f} s) and to "
+            f"the {2 * n * 6 + 1} renders of a central-difference gradient (one image + 2 per parameter)", secs)
     return rounded(out)
 
 
@@ -497,11 +588,13 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="skip the batched leg of the headline workload")
     ap.add_argument("--legs", default="cornell_pt,batch64,inverse",
                     help="further workloads to run after the headline one (comma list, or 'none')")
-    ap.add_argument("--pt-samples", type=int, default=16,
+    ap.add_argument("--pt-samples", type=int, default=wl.CORNELL_PT["samples"],
                     help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",
                     help="slots,blocks_per_cu,stack_k,iters_per_graph[,drain_below[,inline_shadows]] for the batched legs "
                          "instead of autotuning")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the legs (profiling runs); prints the legs' JSON, not the bench line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")
     args = ap.parse_args()
@@ -521,27 +614,27 @@ def main():
 
     import pencil_amd as pa
     core = pa.Core(0)
-    threads = cpu_threads()
-    r = leg_stress(core, pa, args)
+    threads, model = host_cpu()
+    log(f"host: {threads} cores usable ({os.cpu_count()} online), {model}")
+    r = None if args.skip_headline else leg_stress(core, pa, args)
     legs = {}
     for name in [x for x in args.legs.split(",") if x and x != "none"]:
         fn = {"cornell_pt": leg_cornell_pt, "batch64": leg_batch64, "inverse": leg_inverse}[name]
         t0 = time.time()
-        legs[name] = fn(core, pa, args, threads)
+        legs[name] = fn(core, pa, args, threads, model)
         log(f"leg {name}: {time.time() - t0:.1f} s")
+    core.close()
+    if r is None:
+        print(json.dumps({"legs": legs}), flush=True)
+        return
     cpu = None
     if not args.no_cpu:
-        try:
-            cpu = cpu_baseline(r["queries"], threads)
-        except Exception as e:   # the baseline is reported, never required
-            log("cpu baseline failed:", e)
-    core.close()
-
-    traffic = None   # HBM bytes per dispatch from the committed rocprofv3 PMC summary of this kernel
-    tpath = os.path.join(ROOT, "profiles", "k_render_bvh_stress_1080p.traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = round(json.load(f)["hbm_bytes_per_dispatch"])
+        if os.path.exists(HARNESS):
+            t0 = time.time()
+            cpu = cpu_stress(r["queries"], threads, model)
+            log(f"cpu baseline (reference): {cpu['value']} Mrays/s, {time.time() - t0:.1f} s")
+        else:
+            log(f"cpu baseline: {HARNESS} not built (make -C oracle ref); reported as missing")
 
     value, cst, info, batched = r["value"], r["cst"], r["info"], r["batched"]
     line = {
@@ -569,8 +662,7 @@ def main():
                    "device_bvh_build": rounded(r["lbvh"]),
                    "adjoint_grad_steps_per_s": (round(legs["inverse"]["grad_steps_per_s"], 3)
                                                 if "inverse" in legs else None)},
-        "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": roofline(r["bytes"], r["kernel_ms"], "k_render_bvh_stress_1080p", "kernel"),
         "cpu_baseline": cpu,
         "legs": legs,
     }

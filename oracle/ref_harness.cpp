@@ -23,6 +23,11 @@
 //                       L2 loss and central FD of it over albedo/emission
 // opts: --width W --height H --samples n --depth d --pathtrace --mis --bounces b
 //       --threads T --h step --no-fd --rows r0 r1
+//       --repeat N   render N times after one load, printing each render's
+//                    seconds (a CPU baseline: parse and BVH build untimed,
+//                    as src/pencil.cpp:311-313 times render() only)
+//       --no-tag     leave the primitives unwrapped (the reference's own
+//                    objects; ids are then not recorded -- for timing)
 
 #include <algorithm>
 #include <atomic>
@@ -64,7 +69,8 @@ struct Tagged : Primitive {
 
 struct Opts {
     int width = 0, height = 0, samples = 0, depth = 0, threads = 0, bounces = 4;
-    int row0 = -1, row1 = -1;
+    int row0 = -1, row1 = -1, repeat = 1;
+    bool tag = true;
     bool pathtrace = false, fd = true, mis = false;
     float h = 1e-3f;
 };
@@ -85,6 +91,8 @@ static Opts parse_opts(int argc, char** argv, int from) {
         else if (s == "--h") o.h = (float)atof(nxt());
         else if (s == "--no-fd") o.fd = false;
         else if (s == "--rows") { o.row0 = atoi(nxt()); o.row1 = atoi(nxt()); }
+        else if (s == "--repeat") o.repeat = std::max(1, atoi(nxt()));
+        else if (s == "--no-tag") o.tag = false;
     }
     return o;
 }
@@ -101,6 +109,7 @@ static bool load(const std::string& path, ParsedScene& ps, const Opts& o) {
     if (o.samples > 0) ps.samples = o.samples;
     if (o.depth > 0) ps.depth = o.depth;
     // wrap primitives and rebuild the BVH over the wrappers
+    if (!o.tag) return true;
     auto& prims = ps.scene.primitives_;
     for (size_t i = 0; i < prims.size(); ++i) prims[i] = std::make_shared<Tagged>(prims[i], (int)i);
     ps.scene.build_bvh();
@@ -131,45 +140,45 @@ static int cmd_render(int argc, char** argv) {
     RenderSettings rs = settings_for(ps, o);
     Renderer r(ps.scene, ps.camera, rs);
     std::vector<Color> px; RenderStats st;
-    if (o.row0 >= 0) {
-        // A band of rows, 16-pixel column tiles handed out to the threads; a
-        // pixel's value does not depend on the tile it is rendered in (its RNG
-        // is seeded from (i, j), src/renderer.hpp:313).
-        const int W = rs.width, H = rs.height;
-        const int r0 = std::max(0, o.row0), r1 = std::min(H, o.row1);
-        if (r1 <= r0) return 2;
-        px.assign((size_t)W * H, Color{0, 0, 0});
-        r.ray_counter_.store(0);
-        std::atomic<int> next{0};
-        const int tiles = (W + 15) / 16;
-        int T = o.threads > 0 ? o.threads : (int)std::thread::hardware_concurrency();
-        if (T <= 0) T = 1;
-        std::vector<std::thread> ws;
+    const int W = rs.width, H = rs.height;
+    const bool band = o.row0 >= 0;
+    const int r0 = band ? std::max(0, o.row0) : 0, r1 = band ? std::min(H, o.row1) : H;
+    if (r1 <= r0) return 2;
+    uint64_t rays = 0;
+    std::cout << "seconds";
+    for (int rep = 0; rep < o.repeat; ++rep) {
         auto t0 = std::chrono::steady_clock::now();
-        for (int t = 0; t < T; ++t)
-            ws.emplace_back([&]() {
-                for (int k; (k = next.fetch_add(1)) < tiles;)
-                    r.render_tile_region(px, W, H, k * 16, r0, std::min(16, W - k * 16), r1 - r0);
-            });
-        for (auto& w : ws) w.join();
+        if (band) {
+            // A band of rows, 16-pixel column tiles handed out to the threads; a
+            // pixel's value does not depend on the tile it is rendered in (its RNG
+            // is seeded from (i, j), src/renderer.hpp:313).
+            px.assign((size_t)W * H, Color{0, 0, 0});
+            r.ray_counter_.store(0);
+            std::atomic<int> next{0};
+            const int tiles = (W + 15) / 16;
+            int T = o.threads > 0 ? o.threads : (int)std::thread::hardware_concurrency();
+            if (T <= 0) T = 1;
+            std::vector<std::thread> ws;
+            for (int t = 0; t < T; ++t)
+                ws.emplace_back([&]() {
+                    for (int k; (k = next.fetch_add(1)) < tiles;)
+                        r.render_tile_region(px, W, H, k * 16, r0, std::min(16, W - k * 16), r1 - r0);
+                });
+            for (auto& w : ws) w.join();
+            rays = r.ray_counter_.load();
+        } else {
+            r.render(px, &st);
+            rays = st.rays_traced;
+        }
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        std::remove(argv[3]);
-        int32_t hdr[2] = {W, r1 - r0};
-        write_all(argv[3], hdr, sizeof(hdr));
-        write_all(argv[3], px.data() + (size_t)r0 * W, (size_t)(r1 - r0) * W * sizeof(Color));
-        uint64_t rays = r.ray_counter_.load();
-        write_all(argv[3], &rays, sizeof(rays));
-        std::cout << "rays " << rays << "\nseconds " << sec << "\n";
-        return 0;
+        std::cout << " " << sec << std::flush;
     }
-    r.render(px, &st);
+    std::cout << "\nrays " << rays << "\n";
     std::remove(argv[3]);
-    int32_t hdr[2] = {rs.width, rs.height};
+    int32_t hdr[2] = {W, r1 - r0};
     write_all(argv[3], hdr, sizeof(hdr));
-    write_all(argv[3], px.data(), px.size() * sizeof(Color));
-    uint64_t rays = st.rays_traced;
+    write_all(argv[3], px.data() + (size_t)r0 * W, (size_t)(r1 - r0) * W * sizeof(Color));
     write_all(argv[3], &rays, sizeof(rays));
-    std::cout << "rays " << rays << "\n";
     return 0;
 }
 
