@@ -878,6 +878,10 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
         }
     }
     ctx->wf_default = cur;
+    // the candidates grew the engine's buffers to the largest slot count
+    // tried (8M slots: ~25 GB); when that is far above the winner's, free
+    // them and let the next job allocate what it needs
+    if (ctx->wf->cap_slots > 2 * cur.slots) ctx->wf->release();
     if (best) *best = to_abi(cur);
     if (best_ms) *best_ms = cur_ms;
     return PCL_OK;

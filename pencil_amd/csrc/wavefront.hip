@@ -1007,9 +1007,14 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         else hipLaunchKernelGGL(k_finish<-1>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
         return hipGetLastError();
     };
-    hipEvent_t e0, e1;
-    WF_TRY(hipEventCreate(&e0));
-    WF_TRY(hipEventCreate(&e1));
+    // timing events, destroyed on every return path
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (hipEvent_t x : e) if (x) (void)hipEventDestroy(x); }
+    } ev;
+    WF_TRY(hipEventCreate(&ev.e[0]));
+    WF_TRY(hipEventCreate(&ev.e[1]));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     WF_TRY(hipEventRecord(e0, stream));
     hipLaunchKernelGGL(k_reset, dim3(16), dim3(256), 0, stream, c.q);
     hipLaunchKernelGGL(k_start, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c, cfg.slots);
@@ -1044,7 +1049,10 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         }
         WF_TRY(issue());   // reuses slot k & 1, already read
         // every iteration retires at least one query of a finite program; a
-        // job still running after this long is a bug, reported, not waited on
+        // job still running after this long is a bug and reported as one,
+        // after the device finishes the (at most two) graph launches in
+        // flight -- each a fixed number of bounded iterations -- so that no
+        // kernel still uses the buffers when the caller gets the error
         if (now_ms() - t_begin > 300000.0) {
             (void)hipDeviceSynchronize();
             err = "wavefront did not drain in 300 s";
@@ -1074,8 +1082,6 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         for (int w : {ST_RAYS, ST_CLOSEST, ST_ANY}) h_st[w] += h_st[(1 + g) * kStatLine + w];
     float ms = 0.0f;
     WF_TRY(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     out.rays = h_st[ST_RAYS];
     out.closest = h_st[ST_CLOSEST];
     out.anyhit = h_st[ST_ANY];

@@ -562,9 +562,10 @@ inline bool parse_pecl_source(const std::string& src, ParsedScene& out, std::vec
             if (out._pending_mesh_file.empty()) {
                 errors.push_back({0, "mesh block requires a file: ... entry"}); ok = false;
             } else {
-                std::string mpath = out._pending_mesh_file;
-                std::ifstream probe(mpath);
-                if (!probe && !out._base_dir.empty()) mpath = out._base_dir + "/" + mpath;
+                // the path as written, relative to the working directory (the
+                // reference does not try the scene's directory for meshes,
+                // unlike includes)
+                const std::string mpath = out._pending_mesh_file;
                 MeshData md; std::string merr;
                 if (!load_obj(mpath, md, merr)) { errors.push_back({0, merr}); ok = false; }
                 else {

@@ -22,6 +22,28 @@ def golden(name):
     return os.path.join(GOLDEN, name)
 
 
+INGEST_SCENES = ["mesh_test", "mesh_bench", "include_demo"]
+
+
+def load_ingest(name):
+    """One of the reference's mesh{} / include example scenes (tests/golden/examples),
+    parsed from tests/golden as the reference resolves their relative paths from
+    its working directory, at the fixture's render options.  Returns the
+    fixture, the host scene and its settings."""
+    import numpy as np
+    import pencil_amd as pa
+    fx = np.load(golden(f"ingest_{name}.npz"))
+    o = str(fx["opts"]).split()
+    kw = {k[2:]: int(v) for k, v in zip(o[::2], o[1::2])}
+    cwd = os.getcwd()
+    try:
+        os.chdir(GOLDEN)
+        hs = pa.HostScene.from_pecl(os.path.join("examples", name + ".pecl"), **kw)
+    finally:
+        os.chdir(cwd)
+    return fx, hs, hs.settings()
+
+
 def img_stats(img, ref):
     """Parity figures of an HDR image against a reference: fraction of pixels
     bit-identical in all channels, RMSE, and the largest 8-bit difference and

@@ -24,7 +24,7 @@ from PIL import Image
 
 import pencil_amd as pa
 import oracle_lib as ol
-from conftest import SCENES, golden, img_stats
+from conftest import INGEST_SCENES, SCENES, golden, img_stats, load_ingest
 from test_oracle import HDR_FIXTURES, render_fixture_inputs
 
 pytestmark = pytest.mark.gpu
@@ -341,3 +341,28 @@ def test_depth_zero_and_one_match_wavefront_and_oracle(core, name, depth):
     with ol.trig_mode(1):
         cimg, _ = osc.render(hs.camera(), s)
     assert np.array_equal(mk, cimg)
+
+
+@pytest.mark.parametrize("name", INGEST_SCENES)
+def test_ingest_scenes_match_reference(core, name, parity_log):
+    """The reference's mesh{} (OBJ) and include example scenes through the host
+    parser and the HIP core: primary-hit ids and t bit-exact, the reference's
+    leaf order, the HDR image at the deterministic-scene bar and bit for bit
+    against the oracle in the device's rounding."""
+    fx, hs, s = load_ingest(name)
+    sid = core.upload(hs.desc())
+    try:
+        assert np.array_equal(core.leaf_order(sid), fx["leaves"])
+        rays = ol.camera_rays(hs.camera(), int(fx["hit_width"]), int(fx["hit_height"]))
+        hits, _ = core.trace_rays(sid, rays)
+        assert np.array_equal(hits["prim_id"], fx["hit_prim_id"])
+        assert np.array_equal(hits["t"], fx["hit_t"])
+        img, st = core.render(sid, hs.camera(), s)
+    finally:
+        core.free_scene(sid)
+    f = img_stats(img, fx["hdr"])
+    with ol.trig_mode(1):
+        oimg, oc = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    parity_log(f"ingest {name} vs reference", rays=st.rays_traced, ref_rays=int(fx["rays"]), **f)
+    assert np.array_equal(img, oimg) and st.rays_traced == oc.rays_traced
+    assert_reference_bar(name, f)

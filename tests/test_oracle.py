@@ -21,7 +21,7 @@ from PIL import Image
 
 import pencil_amd as pa
 import oracle_lib as ol
-from conftest import SCENES, golden
+from conftest import INGEST_SCENES, SCENES, golden, load_ingest
 
 
 def load(name, **kw):
@@ -133,6 +133,34 @@ def test_oracle_stress_full_topology_matches_reference():
     hits = osc.trace_rays(rays)
     assert np.array_equal(hits["prim_id"], fx["full_hit_prim_id"])
     assert np.array_equal(hits["t"], fx["full_hit_t"])
+
+
+@pytest.mark.parametrize("name", INGEST_SCENES)
+def test_ingest_scenes_match_reference(name):
+    """mesh{} (OBJ loader, fan triangulation, fit_mesh) and include through the
+    host parser: the reference's BVH leaf order and split points (so the same
+    triangles in the same insertion order), its primary hits and its HDR image,
+    through the oracle and through the core's host BVH build."""
+    fx, hs, s = load_ingest(name)
+    d = hs.desc()
+    assert d.n_prims == len(fx["leaves"])
+    osc = ol.OracleScene(d)
+    assert np.array_equal(osc.leaf_order(), fx["leaves"])
+    assert np.array_equal(osc.split_points(), fx["splits"])
+    order, info = pa.bvh_leaf_order_host(d)
+    assert np.array_equal(order, fx["leaves"])
+    rays = ol.camera_rays(hs.camera(), int(fx["hit_width"]), int(fx["hit_height"]))
+    hits = osc.trace_rays(rays)
+    assert np.array_equal(hits["prim_id"], fx["hit_prim_id"])
+    assert np.array_equal(hits["t"], fx["hit_t"])
+    rec = osc.hit_records(rays)
+    hit = fx["hit_prim_id"] >= 0
+    assert np.array_equal(rec[hit, 0:3], fx["hit_p"][hit])
+    assert np.array_equal(rec[hit, 3:6], fx["hit_normal"][hit])
+    assert np.array_equal(rec[hit, 7].astype(np.int32), fx["hit_mat"][hit])
+    img, c = osc.render(hs.camera(), s)
+    assert np.array_equal(img, fx["hdr"])
+    assert c.rays_traced == int(fx["rays"])
 
 
 def test_oracle_cornell_pt_bands_match_reference():
