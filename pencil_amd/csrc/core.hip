@@ -94,13 +94,21 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
     if (COUNT) flush_counts(c, ctr);
 }
 
+#ifndef PCL_SHALLOW_WAVES
+#define PCL_SHALLOW_WAVES 4
+#endif
+// Waves per SIMD the register budget allows: Whitted 3, path tracing 4, the
+// frame-free max_depth <= 1 program PCL_SHALLOW_WAVES.
+template <int PT>
+struct RenderWaves { static constexpr int v = PT == 1 ? 4 : PT == 2 ? PCL_SHALLOW_WAVES : 3; };
+
 // One instantiation per tracing mode (PT: Whitted / path), each with the
 // register budget measured fastest for it: Whitted at most 168 VGPRs, 3 waves
 // per SIMD (faster than 183 / 2 waves; 128 / 4 waves is no faster); path
 // tracing 128 VGPRs, 4 waves (the 1M-triangle Cornell frame 1.19x faster
 // than at 168 / 3 waves, and faster than 96 / 5 waves).
 template <bool COUNT, int PT>
-__global__ __launch_bounds__(kBlock, PT == 1 ? 4 : 3) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
+__global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
                                                    unsigned long long* ctr) {
     __shared__ unsigned long long lds[kStackK * kBlock];
     // A block is 128 consecutive pixels of one row, blocks in row-major
@@ -200,7 +208,7 @@ struct ChunkFetch {
 // row segments per wave; a wave takes its next segment as soon as its own
 // lanes are done, independent of the other wave of its block.
 template <bool COUNT, int PT>
-__global__ __launch_bounds__(kBlock, PT == 1 ? 4 : 3) void k_render_p(DevScene S, CamRec C, RSettings st,
+__global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScene S, CamRec C, RSettings st,
                                                                      float* __restrict__ out, unsigned long long* ctr,
                                                                      uint32_t* heads, int n_chunks) {
     __shared__ unsigned long long lds[kStackK * kBlock];
@@ -264,7 +272,7 @@ struct pcl_ctx {
     // kPersistMinChunksPerWave 64-pixel chunks per resident wave), 0 one block
     // per 128 pixels, 1 persistent.  Environment PCL_RENDER_KERNEL overrides.
     int render_kernel = -1;
-    int render_bpc[2] = {0, 0};           // persistent blocks per CU (Whitted, path); 0 = occupancy
+    int render_bpc[3] = {0, 0, 0};        // persistent blocks per CU (Whitted, path, depth <= 1); 0 = occupancy
 };
 
 namespace {
@@ -369,7 +377,8 @@ int pcl_init(int device, pcl_ctx** out) {
     }
     if (hipMalloc(&ctx->d_heads, kRHeadWords * sizeof(uint32_t)) != hipSuccess) return PCL_ERR_OOM;
     if (const char* e = std::getenv("PCL_RENDER_KERNEL")) ctx->render_kernel = std::atoi(e);
-    if (const char* e = std::getenv("PCL_RENDER_BPC")) ctx->render_bpc[0] = ctx->render_bpc[1] = std::atoi(e);
+    if (const char* e = std::getenv("PCL_RENDER_BPC"))
+        ctx->render_bpc[0] = ctx->render_bpc[1] = ctx->render_bpc[2] = std::atoi(e);
     // the device address of this device's scratch array (device is current)
     if (hipGetSymbolAddress((void**)&ctx->d_scratch, HIP_SYMBOL(scratch)) != hipSuccess || !ctx->d_scratch)
         return PCL_ERR_HIP;
@@ -683,9 +692,12 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     const int tiles = ((st.width + 127) / 128) * st.height;   // 128-pixel row segments
     unsigned long long h[kCtrSlots] = {0};
     const int n_chunks = ((st.width + 63) / 64) * st.height;
-    auto pkern = st.pathtrace ? (counts ? k_render_p<true, 1> : k_render_p<false, 1>)
-                              : (counts ? k_render_p<true, 0> : k_render_p<false, 0>);
-    int& bpc = ctx->render_bpc[st.pathtrace ? 1 : 0];
+    // Whitted with max_depth <= 1 has no recursion: the frame-free program
+    const int mode = st.pathtrace ? 1 : (st.max_depth <= 1 ? 2 : 0);
+    auto pkern = mode == 1 ? (counts ? k_render_p<true, 1> : k_render_p<false, 1>)
+               : mode == 2 ? (counts ? k_render_p<true, 2> : k_render_p<false, 2>)
+                           : (counts ? k_render_p<true, 0> : k_render_p<false, 0>);
+    int& bpc = ctx->render_bpc[mode];
     if (bpc <= 0) {
         int nb = 0;
         HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, pkern, kBlock, 0));
@@ -705,8 +717,9 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
         hipLaunchKernelGGL(pkern, dim3(pgrid), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr,
                            ctx->d_heads, n_chunks);
     } else {
-        auto kern = st.pathtrace ? (counts ? k_render<true, 1> : k_render<false, 1>)
-                                 : (counts ? k_render<true, 0> : k_render<false, 0>);
+        auto kern = mode == 1 ? (counts ? k_render<true, 1> : k_render<false, 1>)
+                  : mode == 2 ? (counts ? k_render<true, 2> : k_render<false, 2>)
+                              : (counts ? k_render<true, 0> : k_render<false, 0>);
         hipLaunchKernelGGL(kern, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
     }
     HIP_TRY(ctx, hipGetLastError());

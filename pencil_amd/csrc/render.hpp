@@ -283,6 +283,44 @@ PCL_D f3 whitted(const DevScene& S, const RSettings& st, Ray r, int depth, Rng& 
     }
 }
 
+// Renderer::trace for max_depth <= 1 (ray casting with direct light): the
+// children of a metal or glass hit are depth-0 calls, which count a trace()
+// and return black.  Same float operations as whitted() on these inputs --
+// the same light loop (and its RNG draws), and the children's black folded
+// with the same expressions -- without the frame stack.
+template <class LC>
+PCL_D f3 whitted_shallow(const DevScene& S, const RSettings& st, const Ray& r, int depth, Rng& rng, Stack& stk,
+                         LC& c) {
+    c.rays++;
+    if (depth <= 0) return f3(0, 0, 0);
+    HitRec rec;
+    if (!scene_hit(S, r, 1e-4f, 1e9f, rec, stk, c)) return ld3(S.bg);
+    int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+    load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+    const f3 local = whitted_local(S, st, r, rec, albedo, emission, spec, shin, spec_color, rng, stk, c);
+    const f3 zero(0.0f, 0.0f, 0.0f);   // a depth-0 child's value
+    if (kind == PCL_MAT_METAL && refl > 0.0f) {
+        c.rays++;                                          // trace(reflected, depth - 1)
+        return local * (1.0f - refl) + zero * albedo * refl;
+    }
+    if (kind == PCL_MAT_DIELECTRIC) {   // trace_dielectric: push_glass's TIR test and Fresnel weights
+        const bool entering = rec.front;
+        const float safe = fmaxf(ior, 0.01f);
+        const float eta = entering ? (1.0f / safe) : safe;
+        const f3 dn = r.d.normalized();
+        const float cos_i = fabsf(dn.dot(rec.normal));
+        const float sin_t2 = eta * eta * (1.0f - cos_i * cos_i);
+        c.rays++;                                          // reflection child
+        if (sin_t2 > 1.0f) return zero * albedo;
+        c.rays++;                                          // refraction child
+        const float cos_t = sqrtf(1.0f - sin_t2);
+        const float kr = schlick(entering ? cos_i : cos_t, eta);
+        const f3 g = zero * kr + zero * (1.0f - kr);
+        return g * albedo;
+    }
+    return local;
+}
+
 // Power-heuristic light-sample term of trace_path_mis (renderer.hpp:1264-1302)
 // for an unoccluded sample with ndotl > 0: albedo * I * (ndotl * w_light)
 // + albedo * I * w_brdf, in the reference's operation order.
@@ -383,6 +421,7 @@ PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, F
 
 // render_tile_region's per-pixel body (renderer.hpp:313-387, non-adaptive).
 // PT: 0 = Whitted trace() only, 1 = trace_path() only (st.pathtrace 1 or 2),
+// 2 = Whitted with max_depth <= 1 (whitted_shallow, no frame stack),
 // -1 = chosen per call from st.pathtrace.
 template <int PT = -1, class LC>
 PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F, Stack& stk,
@@ -397,7 +436,8 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
             u += rng.sym() * (0.5f / (W * n));
             v += rng.sym() * (0.5f / (H * n));
             Ray r = camera_ray(C, u, v, rng);
-            if (PT == 1 || (PT < 0 && st.pathtrace)) acc += trace_path(S, st, r, rng, F, stk, c);
+            if (PT == 2) acc += whitted_shallow(S, st, r, st.max_depth, rng, stk, c);
+            else if (PT == 1 || (PT < 0 && st.pathtrace)) acc += trace_path(S, st, r, rng, F, stk, c);
             else acc += whitted(S, st, r, st.max_depth, rng, F, 0, stk, c);
         }
     }
