@@ -218,69 +218,87 @@ PCL_D Ray push_glass(FR* F, int& nf, const Ray& r, const HitRec& rec, const f3& 
     return rr;
 }
 
-// Renderer::trace (renderer.hpp:403-822, in-core branches) from ray r with
-// `depth`, continuing below frames F[0..nf) already pushed by the caller
-// (trace_path's glass case).  Returns the value the outermost call returns.
+// State of one Renderer::trace recursion (renderer.hpp:403-822, in-core
+// branches) between trace() calls: the ray and depth of the next call and the
+// frames of the calls waiting for their children.
+struct WhittedState {
+    Ray r;
+    int depth, nf;
+};
+
+// One trace() call of the recursion -- its closest-hit query, light loop and
+// children -- followed by the return path as far as it goes.  Returns true
+// when the outermost call has returned, with its value in `result`; false
+// when another trace() call (a child or a refraction sibling) is pending in W.
+template <class LC>
+PCL_D bool whitted_step(const DevScene& S, const RSettings& st, WhittedState& W, Rng& rng, Frame* F, Stack& stk,
+                        LC& c, f3& result) {
+    c.rays++;
+    if (W.depth <= 0) {
+        result = f3(0, 0, 0);
+    } else {
+        HitRec rec;
+        if (!scene_hit(S, W.r, 1e-4f, 1e9f, rec, stk, c)) {
+            result = ld3(S.bg);
+        } else {
+            int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+            load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+            f3 local = whitted_local(S, st, W.r, rec, albedo, emission, spec, shin, spec_color, rng, stk, c);
+            if (kind == PCL_MAT_METAL && refl > 0.0f) {
+                Frame& f = F[W.nf++];
+                f.type = FR_METAL; f.albedo = albedo; f.acc = local; f.w0 = refl; f.depth = W.depth;
+                Ray rr;
+                rr.o = rec.p + rec.normal * st.shadow_bias;
+                rr.d = W.r.d.normalized().reflect(rec.normal).normalized();
+                W.r = rr; W.depth = W.depth - 1;
+                return false;
+            } else if (kind == PCL_MAT_DIELECTRIC) {
+                W.r = push_glass(F, W.nf, W.r, rec, albedo, ior, W.depth, st.shadow_bias);
+                W.depth = W.depth - 1;
+                return false;
+            } else {
+                result = local;
+            }
+        }
+    }
+    // return path: fold child results into their parents
+    while (W.nf > 0) {
+        Frame& f = F[W.nf - 1];
+        if (f.type == FR_METAL) {
+            // local * (1 - rho) + (incoming * albedo) * rho   (:652-657)
+            result = f.acc * (1.0f - f.w0) + result * f.albedo * f.w0;
+            --W.nf;
+        } else if (f.type == FR_GLASS_TIR) {
+            result = result * f.albedo;   // :1609-1611
+            --W.nf;
+        } else if (f.type == FR_GLASS_REFL) {
+            f.acc = result;
+            f.type = FR_GLASS_REFR;
+            W.r.o = f.ro; W.r.d = f.rd;
+            W.depth = f.depth - 1;
+            return false;
+        } else {
+            // (refl * kr + refr * kt) * albedo   (:1622-1626)
+            f3 g = f.acc * f.w0 + result * f.w1;
+            result = g * f.albedo;
+            --W.nf;
+        }
+    }
+    return true;
+}
+
+// Renderer::trace from ray r with `depth`, continuing below frames F[0..nf)
+// already pushed by the caller (trace_path's glass case).  Returns the value
+// the outermost call returns.
 template <class LC>
 PCL_D f3 whitted(const DevScene& S, const RSettings& st, Ray r, int depth, Rng& rng, Frame* F, int nf, Stack& stk,
                  LC& c) {
+    WhittedState W;
+    W.r = r; W.depth = depth; W.nf = nf;
     f3 result;
-    while (true) {
-        c.rays++;
-        bool descended = false;
-        if (depth <= 0) {
-            result = f3(0, 0, 0);
-        } else {
-            HitRec rec;
-            if (!scene_hit(S, r, 1e-4f, 1e9f, rec, stk, c)) {
-                result = ld3(S.bg);
-            } else {
-                int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
-                load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
-                f3 local = whitted_local(S, st, r, rec, albedo, emission, spec, shin, spec_color, rng, stk, c);
-                if (kind == PCL_MAT_METAL && refl > 0.0f) {
-                    Frame& f = F[nf++];
-                    f.type = FR_METAL; f.albedo = albedo; f.acc = local; f.w0 = refl; f.depth = depth;
-                    Ray rr;
-                    rr.o = rec.p + rec.normal * st.shadow_bias;
-                    rr.d = r.d.normalized().reflect(rec.normal).normalized();
-                    r = rr; depth = depth - 1; descended = true;
-                } else if (kind == PCL_MAT_DIELECTRIC) {
-                    r = push_glass(F, nf, r, rec, albedo, ior, depth, st.shadow_bias);
-                    depth = depth - 1; descended = true;
-                } else {
-                    result = local;
-                }
-            }
-        }
-        if (descended) continue;
-        // return path: fold child results into their parents
-        bool resumed = false;
-        while (nf > 0) {
-            Frame& f = F[nf - 1];
-            if (f.type == FR_METAL) {
-                // local * (1 - rho) + (incoming * albedo) * rho   (:652-657)
-                result = f.acc * (1.0f - f.w0) + result * f.albedo * f.w0;
-                --nf;
-            } else if (f.type == FR_GLASS_TIR) {
-                result = result * f.albedo;   // :1609-1611
-                --nf;
-            } else if (f.type == FR_GLASS_REFL) {
-                f.acc = result;
-                f.type = FR_GLASS_REFR;
-                r.o = f.ro; r.d = f.rd;
-                depth = f.depth - 1;
-                resumed = true;
-                break;
-            } else {
-                // (refl * kr + refr * kt) * albedo   (:1622-1626)
-                f3 g = f.acc * f.w0 + result * f.w1;
-                result = g * f.albedo;
-                --nf;
-            }
-        }
-        if (!resumed) return result;
+    while (!whitted_step(S, st, W, rng, F, stk, c, result)) {
     }
+    return result;
 }
 
 // Renderer::trace for max_depth <= 1 (ray casting with direct light): the
@@ -342,81 +360,99 @@ PCL_D f3 mis_light_term(const LightRec& L, const f3& albedo, const f3& I, const 
     return contrib_nee * w_light + contrib_brdf * w_brdf;
 }
 
-// Renderer::trace_path (renderer.hpp:829-972); with st.pathtrace == 2,
+// State of one Renderer::trace_path (renderer.hpp:829-972) walk between
+// bounces: the next ray, throughput, radiance so far and bounces left.
+struct PathState {
+    Ray cur;
+    f3 T, R;
+    int depth;
+};
+
+// One iteration of trace_path's bounce loop (its caller checks depth > 0):
+// closest hit, emission, NEE per light, ambient, Russian roulette, the next
+// ray.  Returns false when the walk ends (miss, roulette, glass: the glass
+// subtree runs here as trace_dielectric does).  With st.pathtrace == 2,
 // trace_path_mis (renderer.hpp:1205-1335), which differs only in the light
 // loop (its tangent frame is the diffuse branch's, computed earlier).
 template <class LC>
-PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, Frame* F, Stack& stk, LC& c) {
+PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& rng, Frame* F, Stack& stk, LC& c) {
     const bool mis = st.pathtrace == 2;
-    c.rays++;
     const int bounces = st.bounces;
-    f3 T(1, 1, 1), R(0, 0, 0);
-    int depth = bounces;
-    while (depth > 0) {
-        HitRec rec;
-        if (!scene_hit(S, cur, 1e-4f, 1e9f, rec, stk, c)) { R += T * ld3(S.bg); break; }
-        int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
-        load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
-        R += T * emission;
-        const f3 sn = rec.normal;
-        for (int li = 0; li < S.n_lights; ++li) {
-            const LightRec L = load_light(S.lights, li);
-            const f3 lpos = ld3(L.pos);
-            f3 Ld;
-            float dist = 1.0f;
-            if (L.kind == PCL_LIGHT_POINT) {
-                f3 L_pos = lpos;
-                if (L.radius > 0.0f) L_pos = jitter_light(lpos, L.radius, rec.p, rng);
-                Ld = L_pos - rec.p; dist = Ld.length(); Ld = Ld / dist;
-            } else {
-                Ld = -lpos; dist = 1e9f;
-            }
-            Ray sr;
-            sr.o = rec.p + rec.normal * st.shadow_bias; sr.d = Ld;
-            if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
-            float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
-            if (!mis) R += T * albedo * ld3(L.I) * ndotl;
-            else if (!(ndotl <= 0.0f)) R += T * mis_light_term(L, albedo, ld3(L.I), Ld, dist, ndotl);
-        }
-        R += T * albedo * ld3(S.amb);
-        if (bounces - depth > 3) {   // Russian roulette, :922-926
-            float p = fmaxf(T.x, fmaxf(T.y, T.z));
-            if (rng.unit() > p) break;
-            T = T / p;
-        }
-        f3 nd;
-        f3 NT = T * albedo;
-        if (kind == PCL_MAT_METAL && refl > 0.0f) {
-            f3 rf = cur.d.normalized().reflect(sn).normalized();
-            f3 up = (fabsf(rf.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
-            f3 t1 = rf.cross(up).normalized();
-            f3 t2 = rf.cross(t1).normalized();
-            float r1 = rng.unit() * 0.1f;
-            float r2 = rng.unit() * 6.28f;
-            nd = (rf + t1 * (r1 * cosf_cr(r2)) + t2 * (r1 * sinf_cr(r2))).normalized();
-            NT = T * albedo * refl;
-        } else if (kind == PCL_MAT_DIELECTRIC) {
-            int nf = 0;
-            Ray rr = push_glass(F, nf, cur, rec, albedo, ior, depth, st.shadow_bias);
-            f3 g = whitted(S, st, rr, depth - 1, rng, F, nf, stk, c);
-            R += T * g - T * emission;
-            break;
+    HitRec rec;
+    if (!scene_hit(S, P.cur, 1e-4f, 1e9f, rec, stk, c)) { P.R += P.T * ld3(S.bg); return false; }
+    int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+    load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+    P.R += P.T * emission;
+    const f3 sn = rec.normal;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const LightRec L = load_light(S.lights, li);
+        const f3 lpos = ld3(L.pos);
+        f3 Ld;
+        float dist = 1.0f;
+        if (L.kind == PCL_LIGHT_POINT) {
+            f3 L_pos = lpos;
+            if (L.radius > 0.0f) L_pos = jitter_light(lpos, L.radius, rec.p, rng);
+            Ld = L_pos - rec.p; dist = Ld.length(); Ld = Ld / dist;
         } else {
-            float r1 = rng.unit() * 6.2831853f;
-            float r2 = rng.unit();
-            float r2s = sqrtf(r2);
-            f3 N = sn;
-            f3 up = (fabsf(N.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
-            f3 Tt = up.cross(N).normalized();
-            f3 B = N.cross(Tt).normalized();
-            nd = (Tt * cosf_cr(r1) * r2s + B * sinf_cr(r1) * r2s + N * sqrtf(1.0f - r2)).normalized();
+            Ld = -lpos; dist = 1e9f;
         }
-        cur.o = rec.p + rec.normal * st.shadow_bias;
-        cur.d = nd;
-        T = NT;
-        depth--;
+        Ray sr;
+        sr.o = rec.p + rec.normal * st.shadow_bias; sr.d = Ld;
+        if (!L.emissive && scene_occluded(S, sr, st.shadow_bias, dist - st.shadow_bias, stk, c)) continue;
+        float ndotl = clampf(sn.dot(Ld), 0.0f, 1.0f);
+        if (!mis) P.R += P.T * albedo * ld3(L.I) * ndotl;
+        else if (!(ndotl <= 0.0f)) P.R += P.T * mis_light_term(L, albedo, ld3(L.I), Ld, dist, ndotl);
     }
-    return R;
+    P.R += P.T * albedo * ld3(S.amb);
+    if (bounces - P.depth > 3) {   // Russian roulette, :922-926
+        float p = fmaxf(P.T.x, fmaxf(P.T.y, P.T.z));
+        if (rng.unit() > p) return false;
+        P.T = P.T / p;
+    }
+    f3 nd;
+    f3 NT = P.T * albedo;
+    if (kind == PCL_MAT_METAL && refl > 0.0f) {
+        f3 rf = P.cur.d.normalized().reflect(sn).normalized();
+        f3 up = (fabsf(rf.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
+        f3 t1 = rf.cross(up).normalized();
+        f3 t2 = rf.cross(t1).normalized();
+        float r1 = rng.unit() * 0.1f;
+        float r2 = rng.unit() * 6.28f;
+        nd = (rf + t1 * (r1 * cosf_cr(r2)) + t2 * (r1 * sinf_cr(r2))).normalized();
+        NT = P.T * albedo * refl;
+    } else if (kind == PCL_MAT_DIELECTRIC) {
+        int nf = 0;
+        Ray rr = push_glass(F, nf, P.cur, rec, albedo, ior, P.depth, st.shadow_bias);
+        f3 g = whitted(S, st, rr, P.depth - 1, rng, F, nf, stk, c);
+        P.R += P.T * g - P.T * emission;
+        return false;
+    } else {
+        float r1 = rng.unit() * 6.2831853f;
+        float r2 = rng.unit();
+        float r2s = sqrtf(r2);
+        f3 N = sn;
+        f3 up = (fabsf(N.y) < 0.9f) ? f3(0, 1, 0) : f3(1, 0, 0);
+        f3 Tt = up.cross(N).normalized();
+        f3 B = N.cross(Tt).normalized();
+        nd = (Tt * cosf_cr(r1) * r2s + B * sinf_cr(r1) * r2s + N * sqrtf(1.0f - r2)).normalized();
+    }
+    P.cur.o = rec.p + rec.normal * st.shadow_bias;
+    P.cur.d = nd;
+    P.T = NT;
+    P.depth--;
+    return true;
+}
+
+// Renderer::trace_path (renderer.hpp:829-972) / trace_path_mis.
+template <class LC>
+PCL_D f3 trace_path(const DevScene& S, const RSettings& st, Ray cur, Rng& rng, Frame* F, Stack& stk, LC& c) {
+    c.rays++;
+    PathState P;
+    P.cur = cur; P.T = f3(1, 1, 1); P.R = f3(0, 0, 0); P.depth = st.bounces;
+    while (P.depth > 0) {
+        if (!path_step(S, st, P, rng, F, stk, c)) break;
+    }
+    return P.R;
 }
 
 // render_tile_region's per-pixel body (renderer.hpp:313-387, non-adaptive).
