@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kAdjBlock, 4) void k_adjoint(DevScene S, CamRec C, 
         AdjFrame* F = reinterpret_cast<AdjFrame*>(fbuf);
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-        stk.stride = kAdjBlock;
+        stk.set_stride(kAdjBlock);
         // render_pixel (render.hpp), replayed
         const int Wd = st.width, Hd = st.height;
         const f3 unit(1.0f, 1.0f, 1.0f);

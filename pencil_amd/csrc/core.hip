@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
         r.d = f3(a.w, b.x, b.y);
         Stack st;
         st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-        st.stride = kBlock;
+        st.set_stride(kBlock);
         float tb, ta;
         // an empty interval misses, as the reference's box test decides; a
         // NaN bound is taken as empty (the slab test's max/min would drop it)
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render(DevScene 
         Frame* F = reinterpret_cast<Frame*>(fbuf);
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-        stk.stride = kBlock;
+        stk.set_stride(kBlock);
         const f3 px = render_pixel<PT>(S, C, st, i, j, F, stk, c);
         float* o = out + ((size_t)j * st.width + i) * 3;
         o[0] = px.x; o[1] = px.y; o[2] = px.z;
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScen
     Frame* F = reinterpret_cast<Frame*>(fbuf);
     Stack stk;
     stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-    stk.stride = kBlock;
+    stk.set_stride(kBlock);
     ChunkFetch f((int)((blockIdx.x * kBlock + threadIdx.x) >> 6));
     int chunk;
     while (f.next(heads, n_chunks, chunk)) {

@@ -356,18 +356,21 @@ PCL_D bool slab_test(float lx, float ly, float lz, float hx, float hy, float hz,
 template <int K, int SPILL>
 struct LaneStack {
     PCL_LDS unsigned long long* lds;   // this lane's column base (ref | near << 32)
-    int stride;                        // lanes per block
+    int shift;                         // log2 of the lanes per block (the column stride)
     int sp;
     unsigned long long spill[SPILL];
+    // lanes per block, a power of two: entries are addressed with a shift, not
+    // an integer multiply (a quarter-rate instruction on every push and pop)
+    PCL_D void set_stride(int lanes) { shift = __builtin_ctz((unsigned)lanes); }
     PCL_D void push(int32_t ref, float near) {
-        const int slot = (sp & (K - 1)) * stride;
+        const int slot = (sp & (K - 1)) << shift;
         if (sp >= K) spill[sp - K] = lds[slot];
         lds[slot] = (unsigned long long)(uint32_t)ref | ((unsigned long long)__float_as_uint(near) << 32);
         ++sp;
     }
     PCL_D int32_t pop(float& near) {
         --sp;
-        const int slot = (sp & (K - 1)) * stride;
+        const int slot = (sp & (K - 1)) << shift;
         const unsigned long long e = lds[slot];
         if (sp >= K) lds[slot] = spill[sp - K];
         near = __uint_as_float((uint32_t)(e >> 32));

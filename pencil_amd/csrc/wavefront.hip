@@ -389,7 +389,7 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
     __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
     st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-    st.stride = kWfBlock;
+    st.set_stride(kWfBlock);
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_SHADOW, g)]; };
     const Slots& S = C.sl;
     TravCount<false> cnt;
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
     __shared__ unsigned long long lds[K * kWfBlock];
     LaneStack<K, kStackK + kStackSpill - K> st;
     st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-    st.stride = kWfBlock;
+    st.set_stride(kWfBlock);
     const int cur = C.q.ctr[C_CUR];
     auto len = [&](int g) -> uint32_t { return C.q.ctr[sidx(S_COUNT0 + cur, g)]; };
     const Slots& S = C.sl;
@@ -830,7 +830,7 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
         Frame* F = reinterpret_cast<Frame*>(fbuf);
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
-        stk.stride = kWfBlock;
+        stk.set_stride(kWfBlock);
         const int W = it.st.width;
         Rng rng((uint32_t)m.w);
         const f3 px = render_samples<PT>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F, stk, c);

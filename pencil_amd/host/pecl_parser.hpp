@@ -114,22 +114,22 @@ public:
         return {TokKind::Ident, std::string(1, c), line_};
     }
 
-    // path-like value: quoted string or raw text up to newline/','/'}' (:226-245)
+    // A file-path value: a double-quoted string (everything up to the next
+    // quote, which is consumed) or a bare run ending before the first newline,
+    // comma or closing brace; carriage returns are dropped and trailing blanks
+    // trimmed in both forms (the reference grammar's path values).
     std::string read_path() {
         skip();
-        if (pos_ >= src_.size()) return "";
-        bool quoted = false;
-        if (src_[pos_] == '"') { quoted = true; pos_++; }
+        const bool quoted = pos_ < src_.size() && src_[pos_] == '"';
+        if (quoted) ++pos_;
+        const size_t end = quoted ? src_.find('"', pos_) : src_.find_first_of("\n,}", pos_);
+        const size_t stop = end == std::string::npos ? src_.size() : end;
         std::string out;
-        while (pos_ < src_.size()) {
-            char c = src_[pos_];
-            if (quoted && c == '"') { pos_++; break; }
-            if (!quoted && (c == '\n' || c == ',' || c == '}')) break;
-            if (c == '\r') { pos_++; continue; }
-            out.push_back(c);
-            pos_++;
-        }
-        while (!out.empty() && (out.back() == ' ' || out.back() == '\t')) out.pop_back();
+        for (size_t k = pos_; k < stop; ++k)
+            if (src_[k] != '\r') out += src_[k];
+        pos_ = (quoted && end != std::string::npos) ? end + 1 : stop;
+        const size_t last = out.find_last_not_of(" \t");
+        out.erase(last == std::string::npos ? 0 : last + 1);
         return out;
     }
     int line() const { return line_; }

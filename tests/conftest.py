@@ -44,6 +44,24 @@ def load_ingest(name):
     return fx, hs, hs.settings()
 
 
+RANDOM_SCENES = [f"rand_{k:02d}" for k in range(24)]
+
+
+def load_random(name):
+    """A seeded random scene of tests/golden/random (see make_random_fixtures.py)
+    with the settings its reference render used: (fixture, host scene, settings)."""
+    import numpy as np
+    import pencil_amd as pa
+    fx = np.load(os.path.join(GOLDEN, "random", name + ".npz"))
+    hs = pa.HostScene.from_pecl(os.path.join(GOLDEN, "random", name + ".pecl"))
+    s = hs.settings()
+    o = str(fx["opts"]).split()
+    if "--pathtrace" in o:
+        s.pathtrace = 2 if "--mis" in o else 1
+        s.pathtrace_bounces = int(o[o.index("--bounces") + 1])
+    return fx, hs, s
+
+
 def img_stats(img, ref):
     """Parity figures of an HDR image against a reference: fraction of pixels
     bit-identical in all channels, RMSE, and the largest 8-bit difference and

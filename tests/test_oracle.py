@@ -21,7 +21,7 @@ from PIL import Image
 
 import pencil_amd as pa
 import oracle_lib as ol
-from conftest import INGEST_SCENES, SCENES, golden, load_ingest
+from conftest import INGEST_SCENES, RANDOM_SCENES, SCENES, golden, load_ingest, load_random
 
 
 def load(name, **kw):
@@ -161,6 +161,29 @@ def test_ingest_scenes_match_reference(name):
     img, c = osc.render(hs.camera(), s)
     assert np.array_equal(img, fx["hdr"])
     assert c.rays_traced == int(fx["rays"])
+
+
+@pytest.mark.parametrize("name", RANDOM_SCENES)
+def test_random_scenes_match_reference(name):
+    """24 seeded random scenes over every primitive, material and light kind,
+    Whitted depths 0..7, path tracing and MIS: the reference's HDR image and
+    ray count, bit for bit."""
+    fx, hs, s = load_random(name)
+    assert hs.unsupported() == ""
+    img, c = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    assert np.array_equal(img, fx["hdr"])
+    assert c.rays_traced == int(fx["rays"])
+
+
+def test_random_scenes_are_not_empty():
+    """The random scenes show their geometry: most pixels differ from the
+    background in most scenes."""
+    lit = []
+    for name in RANDOM_SCENES:
+        fx, hs, _ = load_random(name)
+        bg = np.array(hs.desc().background, dtype=np.float32)
+        lit.append(float(np.any(fx["hdr"] != bg, axis=-1).mean()))
+    assert np.median(lit) > 0.3, lit
 
 
 def test_oracle_cornell_pt_bands_match_reference():
