@@ -55,12 +55,26 @@
 using namespace pencil;
 
 static thread_local int g_last_id = -1;
+// Primitive tests per kind (sphere, triangle, cube, plane, cylinder, cone):
+// every Primitive::hit call the traversal makes, closest-hit and occlusion.
+static std::atomic<unsigned long long> g_tests[6];
+
+static int kind_of(const std::shared_ptr<Primitive>& p) {
+    if (std::dynamic_pointer_cast<Sphere>(p)) return 0;
+    if (std::dynamic_pointer_cast<Triangle>(p)) return 1;
+    if (std::dynamic_pointer_cast<Cube>(p)) return 2;
+    if (std::dynamic_pointer_cast<Plane>(p)) return 3;
+    if (std::dynamic_pointer_cast<Cylinder>(p)) return 4;
+    if (std::dynamic_pointer_cast<Cone>(p)) return 5;
+    return -1;
+}
 
 struct Tagged : Primitive {
     std::shared_ptr<Primitive> inner;
-    int id;
-    Tagged(std::shared_ptr<Primitive> p, int i) : inner(std::move(p)), id(i) {}
+    int id, kind;
+    Tagged(std::shared_ptr<Primitive> p, int i) : inner(std::move(p)), id(i), kind(kind_of(inner)) {}
     bool hit(const Ray& r, float t_min, float t_max, HitRecord& out) const override {
+        if (kind >= 0) g_tests[kind].fetch_add(1, std::memory_order_relaxed);
         if (inner->hit(r, t_min, t_max, out)) { g_last_id = id; return true; }
         return false;
     }
@@ -173,7 +187,9 @@ static int cmd_render(int argc, char** argv) {
         const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::cout << " " << sec << std::flush;
     }
-    std::cout << "\nrays " << rays << "\n";
+    std::cout << "\nrays " << rays << "\ntests";
+    for (auto& t : g_tests) std::cout << " " << t.load();   // over all repeats
+    std::cout << "\n";
     std::remove(argv[3]);
     int32_t hdr[2] = {W, r1 - r0};
     write_all(argv[3], hdr, sizeof(hdr));

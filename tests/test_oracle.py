@@ -186,6 +186,28 @@ def test_random_scenes_are_not_empty():
     assert np.median(lit) > 0.3, lit
 
 
+PRIM_TEST_FIXTURES = HDR_FIXTURES + RANDOM_SCENES
+
+
+def count_inputs(name):
+    """Scene and settings of a primitive-test count fixture (tests/golden/prim_tests.json)."""
+    if name.startswith("rand_"):
+        _, hs, s = load_random(name)
+    else:
+        _, hs, s = render_fixture_inputs(name)
+    return hs, s
+
+
+@pytest.mark.parametrize("name", PRIM_TEST_FIXTURES)
+def test_primitive_tests_match_reference(name):
+    """The bytes model's primitive-test counts: the oracle's traversal makes
+    exactly the reference's Primitive::hit calls, per primitive kind."""
+    ref = json.load(open(golden("prim_tests.json")))[name]
+    hs, s = count_inputs(name)
+    _, c = ol.OracleScene(hs.desc()).render(hs.camera(), s)
+    assert [c.prim_tests[k] for k in range(6)] == ref
+
+
 def test_oracle_cornell_pt_bands_match_reference():
     """The path-traced workload at full size (1920x1080, 256 spp, 8 bounces,
     Russian roulette, 1,048,820-triangle mesh): three row bands of the
