@@ -152,6 +152,13 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
     if (i < n) p[i] = v;
 }
 
+// Zeroes two word ranges in one launch (the render's counters and chunk heads).
+__global__ void k_zero2_u64(unsigned long long* a, int na, unsigned long long* b, int nb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) a[i] = 0ull;
+    if (i < nb) b[i] = 0ull;
+}
+
 // ---- persistent render: waves pull 64-pixel chunks ------------------------
 //
 // Chunk c is the 64-pixel row segment c of the image (row-major segments).
@@ -707,10 +714,8 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     const bool persist = ctx->render_kernel == 1 ||
                          (ctx->render_kernel < 0 && n_chunks >= kPersistMinChunksPerWave * 2 * pgrid);
     // counters (and the chunk heads) cleared by a kernel: no memset node if a caller captures the stream
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_ctr, kCtrSlots, 0ull);
-    if (persist)
-        hipLaunchKernelGGL(k_fill_u64, dim3(kRHeadWords / 2 / 64), dim3(64), 0, ctx->stream,
-                           reinterpret_cast<unsigned long long*>(ctx->d_heads), kRHeadWords / 2, 0ull);
+    hipLaunchKernelGGL(k_zero2_u64, dim3(kRHeadWords / 2 / 64), dim3(64), 0, ctx->stream, ctx->d_ctr, kCtrSlots,
+                       reinterpret_cast<unsigned long long*>(ctx->d_heads), persist ? kRHeadWords / 2 : 0);
     HIP_TRY(ctx, hipGetLastError());
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (persist) {
