@@ -80,12 +80,12 @@ def log(*a):
 def cfg_dict(c):
     return {"slots": c.slots, "blocks_per_cu": c.blocks_per_cu, "stack_k": c.stack_k,
             "iters_per_graph": c.iters_per_graph, "drain_below": c.drain_below,
-            "inline_shadows": c.inline_shadows}
+            "inline_shadows": c.inline_shadows, "streams": c.streams}
 
 
 def cfg_str(c):
     return (f"slots {c.slots}, blocks/CU {c.blocks_per_cu}, stack {c.stack_k}, iters/graph {c.iters_per_graph}, "
-            f"drain below {c.drain_below}, inline shadows {c.inline_shadows}")
+            f"drain below {c.drain_below}, inline shadows {c.inline_shadows}, streams {c.streams}")
 
 
 def host_cpu():
@@ -658,7 +658,7 @@ def main():
     ap.add_argument("--pt-samples", type=int, default=wl.CORNELL_PT["samples"],
                     help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",
-                    help="slots,blocks_per_cu,stack_k,iters_per_graph[,drain_below[,inline_shadows]] for the batched legs "
+                    help="slots,blocks_per_cu,stack_k,iters_per_graph[,drain_below[,inline_shadows[,streams]]] for the batched legs "
                          "instead of autotuning")
     ap.add_argument("--inverse-conditioning", type=int, default=1,
                     help="1: the inverse leg also reports the Gauss-Newton conditioning at the true parameters")

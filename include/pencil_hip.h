@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 5
+#define PCL_ABI_VERSION 6
 
 enum {
     PCL_OK              = 0,
@@ -257,6 +257,12 @@ typedef struct pcl_wf_config {
                                  per-pixel launch finishes the remaining samples; -1 = never */
     int32_t inline_shadows;   /* a vertex with at most this many shadow rays traces them in the
                                  closest-hit kernel instead of the shadow queue; -1 = never */
+    int32_t streams;          /* engines the job's 64-pixel tiles are dealt to (interleaved), each
+                                 with its own stream, slots and graph, run concurrently so that one
+                                 engine's iteration tails and drain overlap the others' work; the
+                                 slots are split among them.  1..4; 0 = by the job: 1 when every
+                                 item renders one sample per pixel, 3 when an item path-traces
+                                 with >= 16 spp, else 2 */
 } pcl_wf_config;
 
 typedef struct pcl_batch_stats {

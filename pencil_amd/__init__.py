@@ -79,7 +79,7 @@ class BatchItem(C.Structure):
 class WfConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("blocks_per_cu", C.c_int32), ("stack_k", C.c_int32),
                 ("iters_per_graph", C.c_int32), ("drain_below", C.c_int32),
-                ("inline_shadows", C.c_int32)]
+                ("inline_shadows", C.c_int32), ("streams", C.c_int32)]
 
 
 class BatchStats(C.Structure):

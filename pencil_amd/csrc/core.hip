@@ -261,7 +261,7 @@ struct pcl_ctx {
     unsigned long long* d_ctr = nullptr;
     std::string err;
     std::vector<SceneSlot> scenes;
-    pcl::wf::Engine* wf = nullptr;
+    pcl::wf::Tracer* wf = nullptr;
     pcl::wf::Config wf_default;      // set by pcl_wf_autotune
     // adjoint work buffers, grown on demand
     double* d_adj_partial = nullptr;
@@ -392,7 +392,7 @@ int pcl_init(int device, pcl_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cus = prop.multiProcessorCount;
-    ctx->wf = new pcl::wf::Engine();
+    ctx->wf = new pcl::wf::Tracer();
     *out = ctx.release();
     return PCL_OK;
 }
@@ -807,6 +807,7 @@ static pcl_wf_config to_abi(const pcl::wf::Config& c) {
     o.slots = c.slots; o.blocks_per_cu = c.blocks_per_cu; o.stack_k = c.stack_k; o.iters_per_graph = c.iters_per_graph;
     o.drain_below = c.drain_below;
     o.inline_shadows = c.inline_shadows;
+    o.streams = c.streams;
     return o;
 }
 
@@ -826,10 +827,13 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
         if (cfg->iters_per_graph > 0) c.iters_per_graph = cfg->iters_per_graph;
         if (cfg->drain_below != 0) c.drain_below = cfg->drain_below;
         if (cfg->inline_shadows != 0) c.inline_shadows = cfg->inline_shadows;
+        if (cfg->streams < 0 || cfg->streams > pcl::wf::kMaxParts)
+            return fail(ctx, PCL_ERR_INVALID, "streams must be 0..4");
+        if (cfg->streams > 0) c.streams = cfg->streams;
     }
     pcl::wf::RunStats rs;
     std::string err;
-    rc = ctx->wf->run(ctx->stream, its, scs, c, ctx->cus, rs, err);
+    rc = ctx->wf->run(ctx->device, ctx->stream, its, scs, c, ctx->cus, rs, err);
     if (rc) return fail(ctx, rc, "wavefront: " + err);
     if (stats) {
         stats->rays_traced = rs.rays;
@@ -858,7 +862,7 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
         pcl::wf::RunStats rs;
         float bestv = 1e30f;
         for (int rep = 0; rep < 2; ++rep) {   // first run builds the graph
-            int r = ctx->wf->run(ctx->stream, its, scs, c, ctx->cus, rs, err);
+            int r = ctx->wf->run(ctx->device, ctx->stream, its, scs, c, ctx->cus, rs, err);
             if (r) return r;
             if (rep > 0 && rs.ms < bestv) bestv = rs.ms;
         }
@@ -881,15 +885,16 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     // 2x2 spp it picked 48 waves' worth of lanes per CU, 3% slower at 256 spp
     // than the rule, the traversal grids' lanes).  Shadow rays per vertex
     // traced inline: none, one, or up to 4 / 32.
+    // Concurrent engines: one, two or three.
     const std::vector<int> inline_opts{-1, 1, 4, 32};
-    for (int knob = 0; knob < 5; ++knob) {
+    for (int knob = 0; knob < 6; ++knob) {
         std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 6, 10, 12, 16}
                               : knob == 2 ? std::vector<int>{8, 16, 32} : knob == 3 ? std::vector<int>{1, 4, 16}
-                              : inline_opts;
+                              : knob == 4 ? inline_opts : std::vector<int>{1, 2, 3};
         for (int v : opts) {
             pcl::wf::Config c = cur;
             (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k
-                       : knob == 3 ? c.iters_per_graph : c.inline_shadows) = v;
+                       : knob == 3 ? c.iters_per_graph : knob == 4 ? c.inline_shadows : c.streams) = v;
             float ms;
             if ((rc = time_cfg(c, ms))) return fail(ctx, rc, "autotune: " + err);
             if (ms < cur_ms) { cur_ms = ms; cur = c; }
@@ -899,7 +904,7 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     // the candidates grew the engine's buffers to the largest slot count
     // tried (8M slots: ~25 GB); when that is far above the winner's, free
     // them and let the next job allocate what it needs
-    if (ctx->wf->cap_slots > 2 * cur.slots) ctx->wf->release();
+    if (ctx->wf->cap_slots() > 2 * (size_t)cur.slots) ctx->wf->release();
     if (best) *best = to_abi(cur);
     if (best_ms) *best_ms = cur_ms;
     return PCL_OK;

@@ -35,9 +35,11 @@
 // color is accumulated with the same operations in the same order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "render.hpp"
@@ -344,10 +346,11 @@ __device__ int advance(const Ctx& C, int s, Lane& L, int status, f3 result, uint
 }
 
 // Hands new pixels to every lane whose slot finished one: one fetch per wave
-// for the whole wave's requests.  The job's 64-pixel segments ("tiles") are
-// dealt to the stripes round-robin (tile t to stripe t % kStripes), so while the stripes
-// advance together the whole GPU works on one front of the image, whose BVH
-// working set the caches share.  Loops because a new pixel can finish
+// for the whole wave's requests.  The part's 64-pixel segments ("tiles") are
+// dealt to the stripes round-robin (local tile t to stripe t % kStripes), and
+// the parts of a job take interleaved tiles, so while the stripes and the
+// parts advance together the whole GPU works on one front of the image, whose
+// BVH working set the caches share.  Loops because a new pixel can finish
 // without a query (zero depth / zero bounces) or be a tile's padding.
 __device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t& rays, bool drain, bool pool_left) {
     if (!pool_left) {   // every job pixel is taken: no fetch
@@ -374,7 +377,8 @@ __device__ int refill(const Ctx& C, int s, Lane& L, int a, Fetcher& pf, uint32_t
             } else {
                 const uint32_t r = (uint32_t)__popcll(need & ((1ull << lane_id()) - 1ull));
                 const uint32_t k = base + r;   // the stripe's k-th pixel
-                if (r < got && assign_pixel(C, L, ((int64_t)(k >> 6) * kStripes + g) * 64 + (k & 63))) {
+                const int64_t tile = ((int64_t)(k >> 6) * kStripes + g) * C.q.parts + C.q.part;
+                if (r < got && assign_pixel(C, L, tile * 64 + (k & 63))) {
                     start_sample(C, L, rays);
                     a = advance(C, s, L, ADV_QUERY, f3(0, 0, 0), rays, drain);
                 }
@@ -939,10 +943,18 @@ static void launch_iter(const Ctx& c, bool one, int grid_trav, int grid_other, h
 }
 
 int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
-                const Config& cfg_in, int cus, RunStats& out, std::string& err) {
+                const Config& cfg_in, int cus, RunStats& out, std::string& err, int part, int parts,
+                hipEvent_t done) {
     Config cfg = cfg_in;
-    int64_t total = 0;
-    for (const auto& it : items) total += padded_pixels(it.st.width, it.st.height);
+    int64_t job = 0;
+    for (const auto& it : items) job += padded_pixels(it.st.width, it.st.height);
+    if (job >= (int64_t)1 << 31) { err = "job larger than 2^31 pixels"; return PCL_ERR_LIMIT; }
+    const int64_t total = part_pixels(job, part, parts);
+    if (total <= 0) {
+        out = RunStats{};
+        if (done) WF_TRY(hipEventRecord(done, stream));
+        return PCL_OK;
+    }
     if (cfg.slots <= 0) cfg.slots = (int)std::min<int64_t>(total, 2 << 20);
     cfg.slots = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.slots, total));
     if (cfg.blocks_per_cu <= 0) cfg.blocks_per_cu = 8;
@@ -952,8 +964,6 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     // traversal grids (blocks_per_cu blocks of kWfBlock lanes per CU)
     if (cfg.drain_below == 0) cfg.drain_below = cus * cfg.blocks_per_cu * kWfBlock;
     if (cfg.inline_shadows == 0) cfg.inline_shadows = 1;
-    if (total <= 0) { out = RunStats{}; return PCL_OK; }
-    if (total >= (int64_t)1 << 31) { err = "job larger than 2^31 pixels"; return PCL_ERR_LIMIT; }
     int rc = ensure(cfg.slots, (int)items.size(), err);
     if (rc) return rc;
     std::vector<ItemRec> its = items;
@@ -968,13 +978,16 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     c.sl = sl;
     c.q = q;
     c.q.total_pixels = total;
+    c.q.part = part;
+    c.q.parts = parts;
     const bool one = its.size() == 1;   // part of the graph key through graph_items
     const int grid_trav = cus * cfg.blocks_per_cu;
     const int grid_other = cus * 8;   // 4 waves per SIMD, the register limit of shade / resolve
     const bool graph_ok = (graph_exec && graph_key.slots == cfg.slots && graph_key.blocks_per_cu == cfg.blocks_per_cu &&
                            graph_key.stack_k == cfg.stack_k && graph_key.iters_per_graph == cfg.iters_per_graph &&
                            graph_key.inline_shadows == cfg.inline_shadows &&
-                           graph_items == (int)its.size() && graph_total == total);
+                           graph_items == (int)its.size() && graph_total == total && graph_part == part &&
+                           graph_parts == parts);
     if (!graph_ok) {
         if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
         hipGraph_t g = nullptr;
@@ -991,6 +1004,8 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         graph_key = cfg;
         graph_items = (int)its.size();
         graph_total = total;
+        graph_part = part;
+        graph_parts = parts;
     }
     if (!h_pending)
         WF_TRY(hipHostMalloc((void**)&h_pending, 2 * kStatusWords * sizeof(uint32_t), hipHostMallocDefault));
@@ -1075,6 +1090,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         WF_TRY(finish(q.fin_meta, q.fin_acc, 0, n_finish, stream));
     }
     WF_TRY(hipEventRecord(e1, stream));
+    if (done) WF_TRY(hipEventRecord(done, stream));
     std::vector<unsigned long long> h_st(kStatWords);
     WF_TRY(hipMemcpyAsync(h_st.data(), q.stats, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     WF_TRY(hipStreamSynchronize(stream));
@@ -1090,6 +1106,109 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     out.finished = n_finish;
     out.ms = ms;
     out.cfg = cfg;
+    return PCL_OK;
+}
+
+Tracer::~Tracer() {
+    for (hipStream_t& s : side)
+        if (s) (void)hipStreamDestroy(s);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    for (hipEvent_t& e : ev_done)
+        if (e) (void)hipEventDestroy(e);
+}
+
+void Tracer::release() {
+    for (Engine& e : eng) e.release();
+}
+
+size_t Tracer::cap_slots() const {
+    size_t n = 0;
+    for (const Engine& e : eng) n += (size_t)e.cap_slots;
+    return n;
+}
+
+int Tracer::run(int device, hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
+                const Config& cfg_in, int cus, RunStats& out, std::string& err) {
+    Config cfg = cfg_in;
+    int64_t total = 0;
+    for (const auto& it : items) total += padded_pixels(it.st.width, it.st.height);
+    if (total <= 0) { out = RunStats{}; out.cfg = cfg; return PCL_OK; }
+    // job-level knobs resolved here, then split among the parts
+    if (cfg.slots <= 0) cfg.slots = (int)std::min<int64_t>(total, 2 << 20);
+    cfg.slots = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.slots, total));
+    if (cfg.blocks_per_cu <= 0) cfg.blocks_per_cu = 8;
+    if (cfg.drain_below == 0) cfg.drain_below = cus * cfg.blocks_per_cu * kWfBlock;
+    if (cfg.streams <= 0) {
+        // Concurrent parts pay where iterations repeat: a one-sample job runs
+        // a handful of iterations and is faster as one engine (headline frame
+        // 1.18 ms vs 1.37 ms in two parts); path-traced jobs with >= 16 spp
+        // run best in three (Cornell + mesh 256 spp: 3.17 / 2.87 / 2.73 s in
+        // one / two / three parts), other jobs in two (64 Whitted scenes at 16
+        // spp: 155.5 / 151 / 157 ms).  Four parts are slower than one with
+        // the four hardware queues HIP gives a process by default.
+        bool multi = false, long_paths = false;
+        for (const auto& it : items) {
+            multi |= it.st.samples > 1;
+            long_paths |= it.st.pathtrace && it.st.samples >= 4;
+        }
+        cfg.streams = long_paths ? 3 : multi ? 2 : 1;
+    }
+    cfg.streams = std::min(cfg.streams, kMaxParts);
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.streams, total >> 6));
+    cfg.streams = parts;
+    if (parts == 1) {
+        const int rc = eng[0].run(stream, items, scenes, cfg, cus, out, err);
+        out.cfg.streams = 1;
+        return rc;
+    }
+    Config pc = cfg;
+    pc.slots = std::max(1, (cfg.slots + parts - 1) / parts);
+    if (cfg.drain_below > 0) pc.drain_below = std::max(1, cfg.drain_below / parts);
+    for (int i = 0; i < parts; ++i) {
+        if (i > 0 && !side[i]) WF_TRY(hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking));
+        if (!ev_done[i]) WF_TRY(hipEventCreate(&ev_done[i]));
+    }
+    if (!ev_fork) WF_TRY(hipEventCreate(&ev_fork));
+    // the parts start after everything the caller queued on the context's stream
+    WF_TRY(hipEventRecord(ev_fork, stream));
+    for (int i = 1; i < parts; ++i) WF_TRY(hipStreamWaitEvent(side[i], ev_fork, 0));
+    std::vector<RunStats> rs(parts);
+    std::vector<std::string> errs(parts);
+    std::vector<int> rcs(parts, PCL_OK);
+    std::vector<std::thread> th;
+    th.reserve(parts - 1);
+    for (int i = 1; i < parts; ++i)
+        th.emplace_back([&, i]() {
+            if (hipSetDevice(device) != hipSuccess) { rcs[i] = PCL_ERR_HIP; errs[i] = "hipSetDevice"; return; }
+            rcs[i] = eng[i].run(side[i], items, scenes, pc, cus, rs[i], errs[i], i, parts, ev_done[i]);
+        });
+    rcs[0] = eng[0].run(stream, items, scenes, pc, cus, rs[0], errs[0], 0, parts, ev_done[0]);
+    for (std::thread& t : th) t.join();
+    for (int i = 0; i < parts; ++i)
+        if (rcs[i] != PCL_OK) {
+            // a part that failed may have left work queued: let every stream finish
+            for (int k = 0; k < parts; ++k) (void)hipStreamSynchronize(k ? side[k] : stream);
+            err = "part " + std::to_string(i) + ": " + errs[i];
+            return rcs[i];
+        }
+    out = RunStats{};
+    float ms = 0.0f;
+    for (int i = 0; i < parts; ++i) {
+        out.rays += rs[i].rays;
+        out.closest += rs[i].closest;
+        out.anyhit += rs[i].anyhit;
+        out.iterations = std::max(out.iterations, rs[i].iterations);
+        out.finished += rs[i].finished;
+        out.graph_launches += rs[i].graph_launches;
+        float m = 0.0f;
+        WF_TRY(hipEventElapsedTime(&m, ev_fork, ev_done[i]));
+        ms = std::max(ms, m);
+    }
+    out.ms = ms;
+    out.cfg = rs[0].cfg;
+    out.cfg.slots = cfg.slots;
+    out.cfg.drain_below = cfg.drain_below;
+    out.cfg.streams = parts;
     return PCL_OK;
 }
 

@@ -80,7 +80,11 @@ struct Queues {
     uint32_t* ctr;            // counters, see sidx()
     unsigned long long* stats;
     int32_t qcap, shcap;      // per-stripe capacities
-    int64_t total_pixels;     // job pixels (a multiple of 64)
+    int64_t total_pixels;     // this part's job pixels (a multiple of 64)
+    // The job's 64-pixel tiles are dealt to `parts` engines that run
+    // concurrently on their own streams: this engine's local tile k is job
+    // tile k * parts + part.
+    int32_t part, parts;
 };
 
 struct Config {
@@ -98,7 +102,21 @@ struct Config {
     // shading; vertices with more go through the shadow queue and the any-hit
     // kernel.  0 = auto (1), < 0 = never.
     int inline_shadows = 0;
+    // Engines (each with its own stream, slots, queues and graph) the job's
+    // tiles are dealt to, interleaved; they run concurrently, so one engine's
+    // iteration tails and drain overlap another's iterations.  The slots are
+    // split evenly among them.  0 = auto (1 to 3 by the job's settings), at
+    // most kMaxParts.
+    int streams = 0;
 };
+
+constexpr int kMaxParts = 4;
+
+// Tiles and pixels of part `part` of `parts` in a job of `total` padded pixels.
+inline int64_t part_pixels(int64_t total, int part, int parts) {
+    const int64_t tiles = total >> 6;
+    return tiles > part ? (tiles - part + parts - 1) / parts * 64 : 0;
+}
 
 struct RunStats {
     unsigned long long rays = 0, closest = 0, anyhit = 0, iterations = 0;
@@ -119,6 +137,7 @@ struct Engine {
     Config graph_key;
     int graph_items = 0;
     int64_t graph_total = 0;
+    int graph_part = -1, graph_parts = 0;
     uint32_t* h_pending = nullptr;      // pinned, kStatusWords per graph launch in flight
     hipEvent_t ev_pending[2] = {nullptr, nullptr};
 
@@ -126,7 +145,26 @@ struct Engine {
     void release();
     int alloc(size_t bytes, void** p, std::string& err);
     int ensure(int slots, int items, std::string& err);
+    // Runs part `part` of `parts` of the job on `stream`; `done`, when given,
+    // is recorded on the stream after the part's last kernel.
     int run(hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
+            const Config& cfg, int cus, RunStats& out, std::string& err, int part = 0, int parts = 1,
+            hipEvent_t done = nullptr);
+};
+
+// The engines of a context and the streams of parts 1.. (part 0 runs on the
+// context's stream).  run() deals the job to cfg.streams engines, drives
+// parts 1.. from host threads of their own (each waits on its own stream)
+// and part 0 from the calling thread, and returns when every part is done.
+struct Tracer {
+    Engine eng[kMaxParts];
+    hipStream_t side[kMaxParts] = {};
+    hipEvent_t ev_fork = nullptr, ev_done[kMaxParts] = {};
+
+    ~Tracer();
+    void release();
+    size_t cap_slots() const;
+    int run(int device, hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
             const Config& cfg, int cus, RunStats& out, std::string& err);
 };
 

@@ -79,6 +79,77 @@ def test_drain_finishes_pixels_bit_for_bit(core, slots, drain, inline):
         assert 0 < wst.finished_pixels <= slots
 
 
+@pytest.mark.parametrize("streams", [1, 2, 3, 4])
+def test_concurrent_parts_do_not_change_results(core, streams):
+    """The job's tiles dealt to 1..4 engines on their own streams: path tracing
+    with Russian roulette and a drain in every part, plus a one-tile item, so
+    that some parts get no tile of it and a one-tile job runs as one part."""
+    hs = pa.HostScene.from_pecl(golden("cornell_box.pecl"), width=72, height=40, samples=2)
+    s = hs.settings()
+    s.pathtrace, s.pathtrace_bounces = 1, 8
+    small = pa.HostScene.from_pecl(golden("shapes.pecl"), width=8, height=1, samples=2)
+    sids, bufs = [], []
+    try:
+        jobs, refs, stats = [], [], []
+        for h, st in ((hs, s), (small, small.settings())):
+            sid = core.upload(h.desc())
+            sids.append(sid)
+            img, mst = core.render(sid, h.camera(), st)
+            d = core.alloc(st.width * st.height * 12)
+            bufs.append((d, st))
+            jobs.append((sid, h.camera(), st, d))
+            refs.append(img)
+            stats.append(mst)
+        cfg = pa.WfConfig(slots=1500, blocks_per_cu=2, stack_k=16, iters_per_graph=2, drain_below=900,
+                          streams=streams)
+        bs = core.render_batch(jobs, cfg)
+        assert bs.used.streams == streams
+        assert bs.rays_traced == sum(m.rays_traced for m in stats)
+        assert bs.closest_queries == sum(m.closest_queries for m in stats)
+        assert bs.anyhit_queries == sum(m.anyhit_queries for m in stats)
+        for (d, st), ref in zip(bufs, refs):
+            out = np.empty((st.height, st.width, 3), dtype=np.float32)
+            core.d2h(out, d)
+            assert np.array_equal(out, ref)
+        one, _ = core.render_wavefront(sids[1], small.camera(), small.settings(), pa.WfConfig(streams=streams))
+        assert np.array_equal(one, refs[1])
+        _, ost = core.render_wavefront(sids[1], small.camera(), small.settings(), pa.WfConfig(streams=streams))
+        assert ost.used.streams == 1   # one tile: one part
+    finally:
+        for d, _ in bufs:
+            core.free(d)
+        for sid in sids:
+            core.free_scene(sid)
+
+
+def test_streams_out_of_range_is_an_error(core):
+    hs = pa.HostScene.from_pecl(golden("shapes.pecl"), width=16, height=16, samples=1)
+    sid = core.upload(hs.desc())
+    try:
+        for bad in (-1, 5):
+            with pytest.raises(pa.CoreError, match="streams"):
+                core.render_wavefront(sid, hs.camera(), hs.settings(), pa.WfConfig(streams=bad))
+    finally:
+        core.free_scene(sid)
+
+
+def test_default_parts_follow_the_job(core):
+    """streams 0: one part for a one-sample job, two for Whitted with more
+    samples, three for path tracing with >= 16 spp."""
+    hs = pa.HostScene.from_pecl(golden("three_spheres.pecl"), width=256, height=64, samples=1)
+    sid = core.upload(hs.desc())
+    try:
+        for samples, pt, want in ((1, 0, 1), (2, 0, 2), (4, 0, 2), (2, 1, 2), (4, 1, 3)):
+            s = hs.settings()
+            s.samples, s.pathtrace, s.pathtrace_bounces = samples, pt, 3
+            mk, _ = core.render(sid, hs.camera(), s)
+            wf, st = core.render_wavefront(sid, hs.camera(), s)
+            assert st.used.streams == want, (samples, pt)
+            assert np.array_equal(wf, mk)
+    finally:
+        core.free_scene(sid)
+
+
 def test_batch_of_uneven_scenes_equals_individual_renders(core):
     """Several scenes of different size, resolution and mode in one job."""
     jobs, refs, bufs, sids = [], [], [], []
