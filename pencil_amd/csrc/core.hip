@@ -20,7 +20,9 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -360,11 +362,18 @@ void fill_stats(pcl_stats* st, const unsigned long long* h, float kms, double t0
 
 }  // namespace
 
+// Host allocations (and host threads) can throw; no exception crosses the C
+// ABI: an entry point's function-try-block turns one into an error code.
+#define PCL_CATCH(ctx)                                                                                   \
+    catch (const std::bad_alloc&) { return fail((ctx), PCL_ERR_OOM, "host allocation failed"); }          \
+    catch (const std::exception& e_) { return fail((ctx), PCL_ERR_HIP, std::string("host: ") + e_.what()); } \
+    catch (...) { return fail((ctx), PCL_ERR_HIP, "host exception"); }
+
 extern "C" {
 
 int pcl_abi_version(void) { return PCL_ABI_VERSION; }
 
-int pcl_init(int device, pcl_ctx** out) {
+int pcl_init(int device, pcl_ctx** out) try {
     if (!out) return PCL_ERR_INVALID;
     *out = nullptr;
     int n = 0;
@@ -395,7 +404,7 @@ int pcl_init(int device, pcl_ctx** out) {
     ctx->wf = new pcl::wf::Tracer();
     *out = ctx.release();
     return PCL_OK;
-}
+} PCL_CATCH(nullptr)
 
 void pcl_destroy(pcl_ctx* ctx) {
     if (!ctx) return;
@@ -494,7 +503,7 @@ static int build_on_device(pcl_ctx* ctx, SceneSlot& sl, HostLayout& L) {
     return PCL_OK;
 }
 
-int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_out) {
+int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_out) try {
     if (!ctx || !desc || !scene_out) return fail(ctx, PCL_ERR_INVALID, "null argument");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const double t0 = now_ms();
@@ -552,7 +561,7 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     sl.info.upload_ms = now_ms() - t1;
     *scene_out = idx;
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 int pcl_scene_free(pcl_ctx* ctx, int32_t scene) {
     SceneSlot* sl = get_scene(ctx, scene);
@@ -570,15 +579,15 @@ int pcl_scene_info_get(pcl_ctx* ctx, int32_t scene, pcl_scene_info* info) {
     return PCL_OK;
 }
 
-int pcl_scene_leaf_order(pcl_ctx* ctx, int32_t scene, int32_t* out, int32_t n) {
+int pcl_scene_leaf_order(pcl_ctx* ctx, int32_t scene, int32_t* out, int32_t n) try {
     SceneSlot* sl = get_scene(ctx, scene);
     if (!sl || !out) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
     if (n < (int32_t)sl->leaf_order.size()) return fail(ctx, PCL_ERR_INVALID, "output too small");
     if (!sl->leaf_order.empty()) std::memcpy(out, sl->leaf_order.data(), sl->leaf_order.size() * sizeof(int32_t));
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
-int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t* leaf_order, int32_t n) {
+int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t* leaf_order, int32_t n) try {
     if (!desc) return PCL_ERR_INVALID;
     const double t0 = now_ms();
     HostLayout L;
@@ -603,7 +612,7 @@ int pcl_bvh_build_host(const pcl_scene_desc* desc, pcl_scene_info* info, int32_t
         if (!L.leaf_order.empty()) std::memcpy(leaf_order, L.leaf_order.data(), L.leaf_order.size() * sizeof(int32_t));
     }
     return PCL_OK;
-}
+} PCL_CATCH(nullptr)
 
 static int launch_trace(pcl_ctx* ctx, SceneSlot* sl, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
                         pcl_hit* d_hits, bool count) {
@@ -620,7 +629,7 @@ static int launch_trace(pcl_ctx* ctx, SceneSlot* sl, const pcl_ray* d_rays, int3
 }
 
 int pcl_trace_rays_device(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
-                          pcl_hit* d_hits, pcl_stats* stats) {
+                          pcl_hit* d_hits, pcl_stats* stats) try {
     SceneSlot* sl = get_scene(ctx, scene);
     if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
     if (n < 0 || (n > 0 && (!d_rays || !d_hits))) return fail(ctx, PCL_ERR_INVALID, "bad ray buffers");
@@ -640,10 +649,10 @@ int pcl_trace_rays_device(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, in
     if (any_hit) h[CTR_ANY] = (unsigned long long)n; else h[CTR_CLOSEST] = (unsigned long long)n;
     fill_stats(stats, h, kms, t0);
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 int pcl_trace_rays(pcl_ctx* ctx, int32_t scene, const pcl_ray* rays, int32_t n, int32_t any_hit, pcl_hit* hits,
-                   pcl_stats* stats) {
+                   pcl_stats* stats) try {
     SceneSlot* sl = get_scene(ctx, scene);
     if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
     if (n < 0 || (n > 0 && (!rays || !hits))) return fail(ctx, PCL_ERR_INVALID, "bad ray buffers");
@@ -659,7 +668,7 @@ int pcl_trace_rays(pcl_ctx* ctx, int32_t scene, const pcl_ray* rays, int32_t n, 
     (void)hipFree(dr);
     (void)hipFree(dh);
     return rc;
-}
+} PCL_CATCH(ctx)
 
 // Work counters for a ray batch: internal nodes entered and primitive tests,
 // summed over the batch (a separate instrumented launch).
@@ -670,7 +679,7 @@ static void fill_counts(pcl_work_counts* out, const unsigned long long* h) {
 }
 
 int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, int32_t n, int32_t any_hit,
-                          pcl_hit* d_hits, pcl_work_counts* counts) {
+                          pcl_hit* d_hits, pcl_work_counts* counts) try {
     SceneSlot* sl = get_scene(ctx, scene);
     if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
     if (n <= 0 || !d_rays || !d_hits) return fail(ctx, PCL_ERR_INVALID, "bad ray buffers");
@@ -683,7 +692,7 @@ int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, in
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     fill_counts(counts, h);
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                        float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) {
@@ -739,18 +748,18 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
 }
 
 int pcl_render_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
-                      float* d_out_rgb, pcl_stats* stats) {
+                      float* d_out_rgb, pcl_stats* stats) try {
     return render_impl(ctx, scene, cam, settings, d_out_rgb, stats, nullptr);
-}
+} PCL_CATCH(ctx)
 
 int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
-                      float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) {
+                      float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) try {
     pcl_work_counts tmp;
     return render_impl(ctx, scene, cam, settings, d_out_rgb, stats, counts ? counts : &tmp);
-}
+} PCL_CATCH(ctx)
 
 int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings, float* out_rgb,
-               pcl_stats* stats) {
+               pcl_stats* stats) try {
     if (!out_rgb) return fail(ctx, PCL_ERR_INVALID, "null output");
     int rc = check_settings(ctx, settings);
     if (rc) return rc;
@@ -765,7 +774,7 @@ int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_set
     (void)hipFree(d);
     if (!rc && stats) stats->total_ms = now_ms() - t0;
     return rc;
-}
+} PCL_CATCH(ctx)
 
 static int batch_prepare(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, std::vector<pcl::wf::ItemRec>& its,
                          std::vector<DevScene>& scs) {
@@ -812,7 +821,7 @@ static pcl_wf_config to_abi(const pcl::wf::Config& c) {
 }
 
 int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const pcl_wf_config* cfg,
-                     pcl_batch_stats* stats) {
+                     pcl_batch_stats* stats) try {
     const double t0 = now_ms();
     std::vector<pcl::wf::ItemRec> its;
     std::vector<DevScene> scs;
@@ -846,9 +855,9 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
         stats->finished_pixels = rs.finished;
     }
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
-int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best, double* best_ms) {
+int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best, double* best_ms) try {
     std::vector<pcl::wf::ItemRec> its;
     std::vector<DevScene> scs;
     int rc = batch_prepare(ctx, items, n, its, scs);
@@ -908,7 +917,7 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     if (best) *best = to_abi(cur);
     if (best_ms) *best_ms = cur_ms;
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 // Grows the context's HDR staging buffers (forward image, target) to n floats.
 static int img_buffers(pcl_ctx* ctx, size_t n) {
@@ -942,7 +951,7 @@ int pcl_tonemap_rgb8_device(pcl_ctx* ctx, const float* d_hdr, int64_t n_px, floa
 }
 
 int pcl_render_rgb8(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
-                    float exposure, uint8_t* out_rgb8, pcl_stats* stats) {
+                    float exposure, uint8_t* out_rgb8, pcl_stats* stats) try {
     if (!out_rgb8) return fail(ctx, PCL_ERR_INVALID, "null output");
     int rc = check_settings(ctx, settings);
     if (rc) return rc;
@@ -965,9 +974,9 @@ int pcl_render_rgb8(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pc
     HIP_TRY(ctx, hipMemcpy(out_rgb8, ctx->d_rgb8, n_px * 3, hipMemcpyDeviceToHost));
     if (stats) stats->total_ms = now_ms() - t0;
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
-int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* materials, int32_t n) {
+int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* materials, int32_t n) try {
     SceneSlot* sl = get_scene(ctx, scene);
     if (!sl) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
     if (!materials || n != sl->n_mats)
@@ -982,7 +991,7 @@ int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* mat
                                 hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 // Grows the adjoint's partials buffer to this launch and makes sure the
 // output buffer exists.
@@ -1043,7 +1052,7 @@ static int adj_run(pcl_ctx* ctx, SceneSlot* sl, const pcl_camera* cam, const pcl
 }
 
 int pcl_render_vjp_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
-                          const float* d_adj, double* grad, pcl_stats* stats) {
+                          const float* d_adj, double* grad, pcl_stats* stats) try {
     SceneSlot* sl = get_scene(ctx, scene);
     int rc = adj_check(ctx, sl, cam, settings, grad);
     if (rc) return rc;
@@ -1051,10 +1060,10 @@ int pcl_render_vjp_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, co
     const double t0 = now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     return adj_run(ctx, sl, cam, settings, nullptr, d_adj, nullptr, grad, nullptr, stats, t0);
-}
+} PCL_CATCH(ctx)
 
 int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
-                       const float* d_target, double* grad, double* loss, float* d_out_rgb, pcl_stats* stats) {
+                       const float* d_target, double* grad, double* loss, float* d_out_rgb, pcl_stats* stats) try {
     SceneSlot* sl = get_scene(ctx, scene);
     int rc = adj_check(ctx, sl, cam, settings, grad);
     if (rc) return rc;
@@ -1062,14 +1071,14 @@ int pcl_adjoint_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     const double t0 = now_ms();
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     return adj_run(ctx, sl, cam, settings, d_target, nullptr, d_out_rgb, grad, loss, stats, t0);
-}
+} PCL_CATCH(ctx)
 
 // pcl_adjoint_device with every work buffer in the device's scratch array:
 // no allocation, the same kernels.  The counters are cleared with a kernel
 // (a capturable node); partials and result are written in full by the pass.
 int pcl_adjoint_device_noalloc(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                                const float* d_target, double* grad, double* loss, float* d_out_rgb,
-                               pcl_stats* stats) {
+                               pcl_stats* stats) try {
     SceneSlot* sl = get_scene(ctx, scene);
     int rc = adj_check(ctx, sl, cam, settings, grad);
     if (rc) return rc;
@@ -1109,10 +1118,10 @@ int pcl_adjoint_device_noalloc(pcl_ctx* ctx, int32_t scene, const pcl_camera* ca
         fill_stats(stats, h, ms, t0);
     }
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
-                const float* target_rgb, double* grad, double* loss, float* out_rgb, pcl_stats* stats) {
+                const float* target_rgb, double* grad, double* loss, float* out_rgb, pcl_stats* stats) try {
     SceneSlot* sl = get_scene(ctx, scene);
     int rc = adj_check(ctx, sl, cam, settings, grad);
     if (rc) return rc;
@@ -1127,7 +1136,7 @@ int pcl_adjoint(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_se
     if (out_rgb) HIP_TRY(ctx, hipMemcpy(out_rgb, ctx->d_img, n * sizeof(float), hipMemcpyDeviceToHost));
     if (stats) stats->total_ms = now_ms() - t0;
     return PCL_OK;
-}
+} PCL_CATCH(ctx)
 
 int pcl_device_alloc(pcl_ctx* ctx, size_t bytes, void** d_ptr) {
     if (!ctx || !d_ptr) return fail(ctx, PCL_ERR_INVALID, "null argument");

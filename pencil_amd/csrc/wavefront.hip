@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1175,14 +1176,32 @@ int Tracer::run(int device, hipStream_t stream, const std::vector<ItemRec>& item
     std::vector<RunStats> rs(parts);
     std::vector<std::string> errs(parts);
     std::vector<int> rcs(parts, PCL_OK);
+    // no exception may leave while a part's thread still runs (nor cross the C ABI)
+    auto run_part = [&](int i) {
+        try {
+            rcs[i] = eng[i].run(i ? side[i] : stream, items, scenes, pc, cus, rs[i], errs[i], i, parts, ev_done[i]);
+        } catch (const std::bad_alloc&) {
+            rcs[i] = PCL_ERR_OOM;
+            errs[i] = "host allocation failed";
+        } catch (...) {
+            rcs[i] = PCL_ERR_HIP;
+            errs[i] = "host exception";
+        }
+    };
     std::vector<std::thread> th;
-    th.reserve(parts - 1);
-    for (int i = 1; i < parts; ++i)
-        th.emplace_back([&, i]() {
-            if (hipSetDevice(device) != hipSuccess) { rcs[i] = PCL_ERR_HIP; errs[i] = "hipSetDevice"; return; }
-            rcs[i] = eng[i].run(side[i], items, scenes, pc, cus, rs[i], errs[i], i, parts, ev_done[i]);
-        });
-    rcs[0] = eng[0].run(stream, items, scenes, pc, cus, rs[0], errs[0], 0, parts, ev_done[0]);
+    std::vector<int> here;   // parts whose thread could not be started: run on this thread
+    for (int i = 1; i < parts; ++i) {
+        try {
+            th.emplace_back([&, i]() {
+                if (hipSetDevice(device) != hipSuccess) { rcs[i] = PCL_ERR_HIP; errs[i] = "hipSetDevice"; return; }
+                run_part(i);
+            });
+        } catch (...) {
+            here.push_back(i);
+        }
+    }
+    run_part(0);
+    for (int i : here) run_part(i);
     for (std::thread& t : th) t.join();
     for (int i = 0; i < parts; ++i)
         if (rcs[i] != PCL_OK) {

@@ -8,6 +8,7 @@
 #include "stress_scene.hpp"
 
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <string>
 
@@ -29,11 +30,20 @@ void copy_err(const std::string& s, char* buf, int len) {
 
 }  // namespace
 
+// The parser converts numbers with std::stof, as the reference's does, so a
+// malformed scene can throw; no exception crosses the C boundary.
+#define PCLH_CATCH_PTR(buf, len)                                                             \
+    catch (const std::exception& e_) { copy_err(std::string("error: ") + e_.what(), buf, len); return nullptr; } \
+    catch (...) { copy_err("error", buf, len); return nullptr; }
+#define PCLH_CATCH_INT(buf, len)                                                             \
+    catch (const std::exception& e_) { copy_err(std::string("error: ") + e_.what(), buf, len); return -1; } \
+    catch (...) { copy_err("error", buf, len); return -1; }
+
 extern "C" {
 
 // Parse a .pecl file and apply --width/--height/--samples/--depth overrides
 // the way the CLI's load_scene does (src/pencil.cpp:246-269); 0 = keep.
-void* pclh_load_pecl(const char* path, int width, int height, int samples, int depth, char* err, int errlen) {
+void* pclh_load_pecl(const char* path, int width, int height, int samples, int depth, char* err, int errlen) try {
     auto h = std::make_unique<Handle>();
     std::vector<pencil::ParseError> errs;
     bool ok = pencil::parse_pecl(path, h->ps, errs);
@@ -51,32 +61,32 @@ void* pclh_load_pecl(const char* path, int width, int height, int samples, int d
     for (auto& u : ps.unsupported) h->unsupported += (h->unsupported.empty() ? "" : "; ") + u;
     h->refresh();
     return h.release();
-}
+} PCLH_CATCH_PTR(err, errlen)
 
-void* pclh_make_stress(int freq, int n_spheres, uint32_t seed, int width, int height) {
+void* pclh_make_stress(int freq, int n_spheres, uint32_t seed, int width, int height) try {
     auto h = std::make_unique<Handle>();
     pencil::StressParams P;
     P.freq = freq; P.n_spheres = n_spheres; P.seed = seed; P.width = width; P.height = height;
     pencil::make_stress_scene(P, h->ps);
     h->refresh();
     return h.release();
-}
+} PCLH_CATCH_PTR(nullptr, 0)
 
 // mat_mix = 1 writes the batched workload's scene (pclh_make_mixed).
 int pclh_write_stress(const char* dir, const char* name, int freq, int n_spheres, uint32_t seed, int width,
-                      int height, int mat_mix, char* err, int errlen) {
+                      int height, int mat_mix, char* err, int errlen) try {
     pencil::StressParams P;
     P.freq = freq; P.n_spheres = n_spheres; P.seed = seed; P.width = width; P.height = height;
     P.mat_mix = mat_mix;
     std::string e;
     if (!pencil::write_stress_scene(P, dir, name, e)) { copy_err(e, err, errlen); return -1; }
     return 0;
-}
+} PCLH_CATCH_INT(err, errlen)
 
 // The path-tracing workload: a Cornell box .pecl with the displaced
 // icosphere (frequency freq, seed) placed inside at v * scale + (cx, cy, cz).
 void* pclh_make_cornell_mesh(const char* pecl, int freq, uint32_t seed, float scale, float cx, float cy, float cz,
-                             int width, int height, char* err, int errlen) {
+                             int width, int height, char* err, int errlen) try {
     void* hv = pclh_load_pecl(pecl, width, height, 0, 0, err, errlen);
     if (!hv) return nullptr;
     auto* h = static_cast<Handle*>(hv);
@@ -86,24 +96,24 @@ void* pclh_make_cornell_mesh(const char* pecl, int freq, uint32_t seed, float sc
     pencil::add_mesh(P, h->ps);
     h->refresh();
     return h;
-}
+} PCLH_CATCH_PTR(err, errlen)
 
 // pclh_make_cornell_mesh's scene as files the reference parses: <dir>/<name>.pecl
 // (the base .pecl plus a mesh{} block) and <dir>/<name>.obj.
 int pclh_write_cornell_mesh(const char* pecl, const char* dir, const char* name, int freq, uint32_t seed,
-                            float scale, float cx, float cy, float cz, char* err, int errlen) {
+                            float scale, float cx, float cy, float cz, char* err, int errlen) try {
     pencil::StressParams P;
     P.freq = freq; P.seed = seed;
     P.mesh_scale = scale; P.mesh_offset[0] = cx; P.mesh_offset[1] = cy; P.mesh_offset[2] = cz;
     std::string e;
     if (!pencil::write_mesh_in_scene(P, pecl, dir, name, e)) { copy_err(e, err, errlen); return -1; }
     return 0;
-}
+} PCLH_CATCH_INT(err, errlen)
 
 // One scene of the batched workload: a displaced icosphere of frequency freq
 // (radius 3) and n_spheres seeded spheres with a diffuse / metal / glass mix,
 // the stress scene's camera and a soft point light.
-void* pclh_make_mixed(int freq, int n_spheres, uint32_t seed, int width, int height) {
+void* pclh_make_mixed(int freq, int n_spheres, uint32_t seed, int width, int height) try {
     auto h = std::make_unique<Handle>();
     pencil::StressParams P;
     P.freq = freq; P.n_spheres = n_spheres; P.seed = seed; P.width = width; P.height = height; P.mat_mix = 1;
@@ -111,14 +121,14 @@ void* pclh_make_mixed(int freq, int n_spheres, uint32_t seed, int width, int hei
     h->ps.scene.lights_mut()[0].radius = 0.5f;
     h->refresh();
     return h.release();
-}
+} PCLH_CATCH_PTR(nullptr, 0)
 
 // The drop-in entry itself: pencil::Renderer::render (host/renderer.hpp, the
 // reference's src/renderer.hpp:198 signature) on the scene with settings s
 // and an optional --bokeh override, backend hip on device 0.  out_rgb:
 // W*H*3 floats.  Returns 0, or -1 with the renderer's message in err.
 int pclh_renderer_render(void* hv, const pcl_settings* s, int bokeh_blades, float* out_rgb, uint64_t* counts,
-                         char* err, int errlen) {
+                         char* err, int errlen) try {
     auto* h = static_cast<Handle*>(hv);
     if (!h || !s || !out_rgb) return -1;
     pencil::RenderSettings rs;
@@ -134,7 +144,7 @@ int pclh_renderer_render(void* hv, const pcl_settings* s, int bokeh_blades, floa
     std::memcpy(out_rgb, px.data(), px.size() * sizeof(pencil::Color));
     if (counts) { counts[0] = st.rays_traced; counts[1] = st.closest_queries; counts[2] = st.anyhit_queries; }
     return 0;
-}
+} PCLH_CATCH_INT(err, errlen)
 
 void pclh_free(void* h) { delete static_cast<Handle*>(h); }
 

@@ -108,6 +108,15 @@ def test_parser_reports_errors(tmp_path):
         pa.HostScene.from_pecl(str(p))
 
 
+def test_out_of_range_number_is_an_error_not_a_crash(tmp_path):
+    """std::stof throws on 1e99, as in the reference's parser; the C entry
+    point turns that into an error instead of letting it cross the boundary."""
+    p = tmp_path / "huge.pecl"
+    p.write_text("sphere { pos: <0, 0, 1e99>, rad: 1 }\n")
+    with pytest.raises(RuntimeError, match="stof"):
+        pa.HostScene.from_pecl(str(p))
+
+
 def test_host_tonemap_matches_oracle():
     rng = np.random.default_rng(0)
     hdr = np.concatenate([rng.uniform(0, 3, (5000, 3)), rng.uniform(0, 1e-3, (500, 3)),
