@@ -179,10 +179,12 @@ const char* pcl_last_error(const pcl_ctx* ctx);
 int         pcl_abi_version(void);
 
 /* Scene upload: builds the SAH BVH with the reference's topology
- * (BVH::build, src/bvh.hpp:54-197) on the host and uploads the flattened
- * 64-byte nodes, primitives in BVH leaf order, materials and lights.
+ * (BVH::build, src/bvh.hpp:54-197) on the host and uploads it as 128-byte
+ * 4-wide nodes, with the primitives in BVH leaf order, materials and lights.
  * Replaces ParsedScene::finalize -> Scene::build_bvh (src/scene.hpp:48-51)
- * for the device.  *scene_out receives a handle >= 0. */
+ * for the device.  *scene_out receives a handle >= 0.  PCL_ERR_LIMIT when
+ * the 4-wide node array would exceed 4 GiB (2^25 nodes, about 67 M
+ * primitives): the traversal addresses it with 32-bit byte offsets. */
 int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_out);
 int pcl_scene_free(pcl_ctx* ctx, int32_t scene);
 int pcl_scene_info_get(pcl_ctx* ctx, int32_t scene, pcl_scene_info* info);

@@ -536,6 +536,13 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) { release(sl); return fail(ctx, PCL_ERR_HIP, hipGetErrorString(e)); }
+    // the traversal addresses nodes with 32-bit byte offsets (intersect.hpp)
+    const int64_t n_wide = device_build ? L.n_wide : (int64_t)L.wide.size();
+    if (n_wide * (int64_t)sizeof(Node4) > ((int64_t)1 << 32)) {
+        release(sl);
+        return fail(ctx, PCL_ERR_LIMIT, std::to_string(n_wide) + " 4-wide BVH nodes: more than the 4 GiB the "
+                                        "traversal addresses");
+    }
     if (L.stack_need > kStackK + kStackSpill) {
         release(sl);
         return fail(ctx, PCL_ERR_LIMIT, "BVH traversal needs " + std::to_string(L.stack_need) +

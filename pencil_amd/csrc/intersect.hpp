@@ -306,9 +306,17 @@ struct DevScene {
 };
 
 // Per-ray slab setup: 1/d per axis as AABB::hit computes it (1.0f / d[a]).
+// The slab's t0/t1 swap when inv_d < 0 (primitives.hpp:41): the entry face
+// of axis a is then the box's max face.  A Node4 holds the four children's
+// min faces lx, ly, lz at bytes 0, 16, 32 and max faces hx, hy, hz at 48,
+// 64, 80; sw[a] (0 or 48) moves axis a's entry-face load from the min to the
+// max face and its exit-face load the other way, so a node step loads each
+// child's entry and exit faces directly instead of swapping two computed
+// distances per axis and child.
 struct SlabRay {
     float ox, oy, oz, ix, iy, iz;
     bool nx, ny, nz;   // inv_d < 0: the slab's t0/t1 swap
+    uint32_t sw[3];
 };
 
 PCL_D SlabRay make_slab_ray(const Ray& r) {
@@ -316,6 +324,9 @@ PCL_D SlabRay make_slab_ray(const Ray& r) {
     s.ox = r.o.x; s.oy = r.o.y; s.oz = r.o.z;
     s.ix = 1.0f / r.d.x; s.iy = 1.0f / r.d.y; s.iz = 1.0f / r.d.z;
     s.nx = s.ix < 0.0f; s.ny = s.iy < 0.0f; s.nz = s.iz < 0.0f;
+    s.sw[0] = s.nx ? 48u : 0u;
+    s.sw[1] = s.ny ? 48u : 0u;
+    s.sw[2] = s.nz ? 48u : 0u;
     return s;
 }
 
@@ -346,6 +357,22 @@ PCL_D bool slab_test(float lx, float ly, float lz, float hx, float hy, float hz,
     if (s.nz) { float x = t0; t0 = t1; t1 = x; }
     t_min = fmaxf(t0, t_min);
     t_max = fminf(t1, t_max);
+    near = t_min;
+    return !(t_max < t_min);
+}
+
+// slab_test with the faces already ordered: (nx, ny, nz) are the entry faces
+// and (fx, fy, fz) the exit faces of the ray's slabs.  t0 = (entry - o) *
+// inv_d and t1 = (exit - o) * inv_d are the swapped (t0, t1) of slab_test
+// bit for bit: the same operands, the same operations.
+PCL_D bool slab_test_ordered(float nx, float ny, float nz, float fx, float fy, float fz, const SlabRay& s,
+                             float t_min, float t_max, float& near) {
+    t_min = fmaxf((nx - s.ox) * s.ix, t_min);
+    t_max = fminf((fx - s.ox) * s.ix, t_max);
+    t_min = fmaxf((ny - s.oy) * s.iy, t_min);
+    t_max = fminf((fy - s.oy) * s.iy, t_max);
+    t_min = fmaxf((nz - s.oz) * s.iz, t_min);
+    t_max = fminf((fz - s.oz) * s.iz, t_max);
     near = t_min;
     return !(t_max < t_min);
 }
@@ -447,15 +474,21 @@ PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, La
     while (true) {
         if (ref >= 0) {
             cnt.node();
-            const float4* np = reinterpret_cast<const float4*>(S.nodes + ref);
-            const float4 lx = gload_f4(np), ly = gload_f4(np + 1), lz = gload_f4(np + 2);
-            const float4 hx = gload_f4(np + 3), hy = gload_f4(np + 4), hz = gload_f4(np + 5);
-            const int4 ch = gload_i4(np + 6);
+            // 32-bit byte offsets from the node array's base (under 4 GiB,
+            // checked at upload): with a uniform base the loads take it from
+            // scalar registers and each address costs one 32-bit add
+            const char* nb = reinterpret_cast<const char*>(S.nodes);
+            const uint32_t off = (uint32_t)ref << 7;
+            const float4 ex = gload_f4(nb + (off + s.sw[0])), ey = gload_f4(nb + (off + s.sw[1] + 16u)),
+                         ez = gload_f4(nb + (off + s.sw[2] + 32u));
+            const float4 fx = gload_f4(nb + (off - s.sw[0] + 48u)), fy = gload_f4(nb + (off - s.sw[1] + 64u)),
+                         fz = gload_f4(nb + (off - s.sw[2] + 80u));
+            const int4 ch = gload_i4(nb + (off + 96u));
             float n0, n1, n2, n3;
-            const bool p0 = slab_test(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, s, t_min, t_max, n0);
-            const bool p1 = slab_test(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, s, t_min, t_max, n1);
-            const bool p2 = slab_test(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, s, t_min, t_max, n2);
-            const bool p3 = slab_test(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, s, t_min, t_max, n3);
+            const bool p0 = slab_test_ordered(ex.x, ey.x, ez.x, fx.x, fy.x, fz.x, s, t_min, t_max, n0);
+            const bool p1 = slab_test_ordered(ex.y, ey.y, ez.y, fx.y, fy.y, fz.y, s, t_min, t_max, n1);
+            const bool p2 = slab_test_ordered(ex.z, ey.z, ez.z, fx.z, fy.z, fz.z, s, t_min, t_max, n2);
+            const bool p3 = slab_test_ordered(ex.w, ey.w, ez.w, fx.w, fy.w, fz.w, s, t_min, t_max, n3);
             if (p3) st.push(ch.w, n3);
             if (p2) st.push(ch.z, n2);
             if (p1) st.push(ch.y, n1);
