@@ -138,18 +138,23 @@ def profile_traffic(name):
     return int(round(t["hbm_bytes_per_dispatch"])), t
 
 
-def roofline(bytes_per_launch, kernel_ms, traffic_name, scope):
+def roofline(bytes_per_launch, kernel_ms, traffic_name, scope, limiter):
     """Algorithmic-byte roofline of a kernel (or a job of kernels), with the
-    counter-measured HBM traffic beside it."""
+    counter-measured HBM traffic, L2 hit rate and VALU issue beside it.
+    `frac` is algorithmic-byte throughput over the HBM peak; `limiter` says
+    what the counters show actually bounds the kernel (none of these is
+    bandwidth-bound: the BVH sits in the 256 MB MALL)."""
     achieved = bytes_per_launch / (kernel_ms / 1000.0) / 1e9
     traffic, t = profile_traffic(traffic_name)
     r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "scope": scope,
-         "algorithmic_bytes": int(bytes_per_launch)}
+         "algorithmic_bytes": int(bytes_per_launch), "limiter": limiter}
     if traffic:
         # the rate the counters see, over the kernel time measured here
         r["measured_hbm_gbs"] = round(traffic / (kernel_ms / 1000.0) / 1e9, 1)
         r["measured_frac"] = round(traffic / (kernel_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4)
+        r["l2_hit"] = round(t["l2_hit"], 3)
+        r["valu_issue"] = round(t["valu_issue"], 3)
         r["traffic_profile"] = f"profiles/{traffic_name}.md"
     return r
 
@@ -254,7 +259,8 @@ def leg_stress(core, pa, args):
                    "mrays": wq / (wms / 1000.0) / 1e6, "rmse_vs_plain": rmse(a, b),
                    "bit_identical_to_plain": bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))),
                    "iterations": bs.iterations, "config": cfg_dict(bs.used),
-                   "roofline": roofline(bytes_per_launch, wk, "wavefront_bvh_stress_1080p", "job")}
+                   "roofline": roofline(bytes_per_launch, wk, "wavefront_bvh_stress_1080p", "job",
+                                         "one iteration: the closest-hit kernel's traversal latency, then resolve")}
         log(f"batched: {wms:.3f} ms/frame wall, {wk:.3f} ms on the stream, "
             f"{batched['mrays']:.1f} Mrays/s, RMSE vs plain {batched['rmse_vs_plain']:.3g}")
         core.free(d_wf)
@@ -353,8 +359,10 @@ def leg_cornell_pt(core, pa, args, threads, model):
            "speedup_batched_vs_plain": plain_ms / wf_ms, "rmse_batched_vs_plain": rmse(a, b),
            "bit_identical": bool(np.array_equal(a.view(np.uint32), b.view(np.uint32))),
            "image_mean": float(a.mean()),
-           "roofline": roofline(nbytes, ps.kernel_ms, "k_render_cornell_pt_1080p_256spp", "kernel"),
-           "batched_roofline": roofline(nbytes, bs.kernel_ms, "wavefront_cornell_pt_1080p_256spp", "job")}
+           "roofline": roofline(nbytes, ps.kernel_ms, "k_render_cornell_pt_1080p_256spp", "kernel",
+                                "VALU issue: divergent lanes (path lengths, glass subtrees)"),
+           "batched_roofline": roofline(nbytes, bs.kernel_ms, "wavefront_cornell_pt_1080p_256spp", "job",
+                                        "traversal latency, iteration barriers and the drain tail")}
     if not args.no_cpu and os.path.exists(HARNESS):
         rows = (H // 2 - 2, H // 2 + 2)
         tmp = tempfile.mkdtemp(prefix="pcl_bench_")
@@ -449,8 +457,10 @@ def leg_batch64(core, pa, args, threads, model):
            "plain_sequential_ms": plain_ms, "plain_sequential_kernel_ms": pk,
            "plain_sequential_mrays_s": pq / plain_ms / 1e3, "speedup_batched_vs_plain": plain_ms / wf_ms,
            "max_rmse_batched_vs_plain": err, "bit_identical": same,
-           "roofline": roofline(nbytes, bs.kernel_ms, "wavefront_batch64_16spp", "job"),
-           "plain_roofline": roofline(nbytes, pk, "k_render_batch64_16spp", "64 kernels")}
+           "roofline": roofline(nbytes, bs.kernel_ms, "wavefront_batch64_16spp", "job",
+                                "traversal latency and iteration barriers"),
+           "plain_roofline": roofline(nbytes, pk, "k_render_batch64_16spp", "64 kernels",
+                                      "fill: a small image cannot occupy the chip")}
     if not args.no_cpu and os.path.exists(HARNESS):
         # a bounded sample: the job's first 4 scenes, each rendered whole by the reference
         sample = list(range(4))
@@ -572,7 +582,8 @@ def leg_inverse(core, pa, args, threads, model, steps=200):
            "loss_at_100": losses[min(99, steps - 1)],
            "param_rmse_start_vs_true": None, "fd_check_at_start": fd_err,
            "fd_check_max_rel_err": max(v["rel_err"] for v in fd_err.values()),
-           "roofline": roofline(nbytes, gpu_ms, "k_adjoint_cornell_512_64spp", "kernel")}
+           "roofline": roofline(nbytes, gpu_ms, "k_adjoint_cornell_512_64spp", "kernel",
+                                "VALU issue and traversal latency of the replayed paths")}
     x0 = np.random.default_rng(7).uniform(0.2, 0.8, true.shape)
     out["param_rmse_last_vs_true"] = float(np.sqrt(np.mean((x - true) ** 2)))
     out["param_rmse_start_vs_true"] = float(np.sqrt(np.mean((x0 - true) ** 2)))
@@ -731,7 +742,8 @@ def main():
                    "device_bvh_build": rounded(r["lbvh"]),
                    "adjoint_grad_steps_per_s": (round(legs["inverse"]["grad_steps_per_s"], 3)
                                                 if "inverse" in legs else None)},
-        "roofline": roofline(r["bytes"], r["kernel_ms"], "k_render_bvh_stress_1080p", "kernel"),
+        "roofline": roofline(r["bytes"], r["kernel_ms"], "k_render_bvh_stress_1080p", "kernel",
+                                "dependent node-fetch latency of the traversal"),
         "cpu_baseline": cpu,
         "legs": legs,
     }

@@ -443,6 +443,35 @@ struct TravCount {
 // in the reference.  Returns the winning slot (-1 on a miss); for closest hits
 // t_best is its t and t_acc the t_max it was accepted with (needed to rebuild
 // its record: a cone's answer depends on t_max).
+// One 4-wide node `ref` of the traversal: slab distances of the four child
+// boxes with the current interval; children 3..1 that pass are pushed with
+// their entry distance; returns true with ref = child 0 when child 0 passes.
+template <int K, int SPILL, class TC>
+PCL_D bool node_step(const DevScene& S, const SlabRay& s, float t_min, float t_max, int32_t& ref,
+                     LaneStack<K, SPILL>& st, TC& cnt) {
+    cnt.node();
+    // 32-bit byte offsets from the node array's base (under 4 GiB, checked at
+    // upload): with a uniform base the loads take it from scalar registers
+    // and each address costs one 32-bit add
+    const char* nb = reinterpret_cast<const char*>(S.nodes);
+    const uint32_t off = (uint32_t)ref << 7;
+    const float4 ex = gload_f4(nb + (off + s.sw[0])), ey = gload_f4(nb + (off + s.sw[1] + 16u)),
+                 ez = gload_f4(nb + (off + s.sw[2] + 32u));
+    const float4 fx = gload_f4(nb + (off - s.sw[0] + 48u)), fy = gload_f4(nb + (off - s.sw[1] + 64u)),
+                 fz = gload_f4(nb + (off - s.sw[2] + 80u));
+    const int4 ch = gload_i4(nb + (off + 96u));
+    float n0, n1, n2, n3;
+    const bool p0 = slab_test_ordered(ex.x, ey.x, ez.x, fx.x, fy.x, fz.x, s, t_min, t_max, n0);
+    const bool p1 = slab_test_ordered(ex.y, ey.y, ez.y, fx.y, fy.y, fz.y, s, t_min, t_max, n1);
+    const bool p2 = slab_test_ordered(ex.z, ey.z, ez.z, fx.z, fy.z, fz.z, s, t_min, t_max, n2);
+    const bool p3 = slab_test_ordered(ex.w, ey.w, ez.w, fx.w, fy.w, fz.w, s, t_min, t_max, n3);
+    if (p3) st.push(ch.w, n3);
+    if (p2) st.push(ch.z, n2);
+    if (p1) st.push(ch.y, n1);
+    if (p0) ref = ch.x;
+    return p0;
+}
+
 template <bool ANY, int K, int SPILL, class TC>
 PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, LaneStack<K, SPILL>& st,
                    float& t_best, float& t_acc, TC& cnt) {
@@ -473,26 +502,7 @@ PCL_D int traverse(const DevScene& S, const Ray& r, float t_min, float t_max, La
     st.sp = 0;
     while (true) {
         if (ref >= 0) {
-            cnt.node();
-            // 32-bit byte offsets from the node array's base (under 4 GiB,
-            // checked at upload): with a uniform base the loads take it from
-            // scalar registers and each address costs one 32-bit add
-            const char* nb = reinterpret_cast<const char*>(S.nodes);
-            const uint32_t off = (uint32_t)ref << 7;
-            const float4 ex = gload_f4(nb + (off + s.sw[0])), ey = gload_f4(nb + (off + s.sw[1] + 16u)),
-                         ez = gload_f4(nb + (off + s.sw[2] + 32u));
-            const float4 fx = gload_f4(nb + (off - s.sw[0] + 48u)), fy = gload_f4(nb + (off - s.sw[1] + 64u)),
-                         fz = gload_f4(nb + (off - s.sw[2] + 80u));
-            const int4 ch = gload_i4(nb + (off + 96u));
-            float n0, n1, n2, n3;
-            const bool p0 = slab_test_ordered(ex.x, ey.x, ez.x, fx.x, fy.x, fz.x, s, t_min, t_max, n0);
-            const bool p1 = slab_test_ordered(ex.y, ey.y, ez.y, fx.y, fy.y, fz.y, s, t_min, t_max, n1);
-            const bool p2 = slab_test_ordered(ex.z, ey.z, ez.z, fx.z, fy.z, fz.z, s, t_min, t_max, n2);
-            const bool p3 = slab_test_ordered(ex.w, ey.w, ez.w, fx.w, fy.w, fz.w, s, t_min, t_max, n3);
-            if (p3) st.push(ch.w, n3);
-            if (p2) st.push(ch.z, n2);
-            if (p1) st.push(ch.y, n1);
-            if (p0) { ref = ch.x; continue; }
+            if (node_step(S, s, t_min, t_max, ref, st, cnt)) continue;
         } else {
             const int slot = ~ref;
             PrimView P = load_prim(S.prims, slot);
