@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 6
+#define PCL_ABI_VERSION 7
 
 enum {
     PCL_OK              = 0,
@@ -141,6 +141,8 @@ typedef struct pcl_stats {
     uint64_t anyhit_queries;    /* any-hit (shadow) BVH queries cast */
     double   kernel_ms;         /* HIP-event time of the launches on the context stream */
     double   total_ms;          /* host wall time of the call */
+    uint64_t deferred_glass;    /* path-traced renders: glass vertices whose Whitted subtree a second
+                                   launch evaluated (scenes without point-light radii; 0 otherwise) */
 } pcl_stats;
 
 /* Ray batches for the traversal entry point. */

@@ -64,7 +64,7 @@ class Settings(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("rays_traced", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
-                ("kernel_ms", C.c_double), ("total_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("total_ms", C.c_double), ("deferred_glass", C.c_uint64)]
 
 
 class WorkCounts(C.Structure):

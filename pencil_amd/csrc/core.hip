@@ -109,9 +109,44 @@ struct RenderWaves { static constexpr int v = PT == 1 ? 4 : PT == 2 ? PCL_SHALLO
 // per SIMD (faster than 183 / 2 waves; 128 / 4 waves is no faster); path
 // tracing 128 VGPRs, 4 waves (the 1M-triangle Cornell frame 1.19x faster
 // than at 168 / 3 waves, and faster than 96 / 5 waves).
-template <bool COUNT, int PT>
+// Buffers of the deferred glass vertices (render_pixel_defer, k_glass, k_sum).
+struct DeferBufs {
+    GlassSink sink;    // tasks, count, cap
+    float* samp;       // sample k of pixel p at (k * n_px + p) * 3
+    float* prefix;     // per pixel: sum of the samples before the first deferred one
+    int32_t* first;    // per pixel: first deferred sample, n^2 when none
+    int64_t n_px;
+};
+
+// The per-pixel body shared by both grids: render_pixel, or with DEFER its
+// deferred-glass form (path tracing only), which leaves the pixel to k_sum
+// when it deferred a sample.
+template <int PT, bool DEFER, class LC>
+__device__ __forceinline__ void shade_pixel(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j,
+                                            Frame* F, Stack& stk, LC& c, float* out, DeferBufs& db) {
+    const int64_t p = (int64_t)j * st.width + i;
+    f3 px;
+    if (DEFER) {
+        int first;
+        db.sink.pixel = (int32_t)p;
+        if (!render_pixel_defer(S, C, st, i, j, F, stk, c, db.sink, db.samp + p * 3, (size_t)db.n_px * 3, px,
+                                first)) {
+            float* q = db.prefix + p * 3;
+            q[0] = px.x; q[1] = px.y; q[2] = px.z;
+            db.first[p] = first;
+            return;
+        }
+        db.first[p] = first;
+    } else {
+        px = render_pixel<PT>(S, C, st, i, j, F, stk, c);
+    }
+    float* o = out + p * 3;
+    o[0] = px.x; o[1] = px.y; o[2] = px.z;
+}
+
+template <bool COUNT, int PT, bool DEFER>
 __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
-                                                   unsigned long long* ctr) {
+                                                   unsigned long long* ctr, DeferBufs db) {
     __shared__ unsigned long long lds[kStackK * kBlock];
     // A block is 128 consecutive pixels of one row, blocks in row-major
     // order: measured 7% faster on the 1M-primitive frame and 16% on the
@@ -130,9 +165,7 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render(DevScene 
         Stack stk;
         stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
         stk.set_stride(kBlock);
-        const f3 px = render_pixel<PT>(S, C, st, i, j, F, stk, c);
-        float* o = out + ((size_t)j * st.width + i) * 3;
-        o[0] = px.x; o[1] = px.y; o[2] = px.z;
+        shade_pixel<PT, DEFER>(S, C, st, i, j, F, stk, c, out, db);
     }
     flush_counts(c, ctr);
 }
@@ -216,10 +249,10 @@ struct ChunkFetch {
 // k_render with a persistent grid: same per-pixel program, pixels in 64-pixel
 // row segments per wave; a wave takes its next segment as soon as its own
 // lanes are done, independent of the other wave of its block.
-template <bool COUNT, int PT>
+template <bool COUNT, int PT, bool DEFER>
 __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScene S, CamRec C, RSettings st,
                                                                      float* __restrict__ out, unsigned long long* ctr,
-                                                                     uint32_t* heads, int n_chunks) {
+                                                                     uint32_t* heads, int n_chunks, DeferBufs db) {
     __shared__ unsigned long long lds[kStackK * kBlock];
     const int segs_x = (st.width + 63) / 64;
     const int lane = threadIdx.x & 63;
@@ -234,13 +267,74 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScen
     while (f.next(heads, n_chunks, chunk)) {
         const int j = chunk / segs_x;
         const int i = (chunk - j * segs_x) * 64 + lane;
-        if (i < st.width) {
-            const f3 px = render_pixel<PT>(S, C, st, i, j, F, stk, c);
-            float* o = out + ((size_t)j * st.width + i) * 3;
-            o[0] = px.x; o[1] = px.y; o[2] = px.z;
-        }
+        if (i < st.width) shade_pixel<PT, DEFER>(S, C, st, i, j, F, stk, c, out, db);
     }
     flush_counts(c, ctr);
+}
+
+// The deferred glass vertices: one lane per task (grid-stride over 64-task
+// chunks), trace_dielectric's Whitted subtree from the recorded frame --
+// whitted() with the frame pushed, as path_step runs it -- then the sample's
+// value R + (T * g - T * emission) into its slot.  Whitted draws no random
+// numbers in a scene without point-light radii (the only scenes deferred),
+// so the generator passed here is never advanced.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock, 3) void k_glass(DevScene S, RSettings st, const float4* __restrict__ tasks,
+                                                     const uint32_t* count, uint32_t cap, float* samp, int64_t n_px,
+                                                     unsigned long long* ctr) {
+    __shared__ unsigned long long lds[kStackK * kBlock];
+    const uint32_t n = min(*count, cap);
+    LaneCount<COUNT> c;
+    alignas(16) unsigned char fbuf[sizeof(Frame) * kMaxFrames];
+    Frame* F = reinterpret_cast<Frame*>(fbuf);
+    Stack stk;
+    stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
+    stk.set_stride(kBlock);
+    const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6, waves = (gridDim.x * kBlock) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
+        const uint32_t t = base + (threadIdx.x & 63);
+        if (t >= n) continue;
+        const float4* q = tasks + (size_t)t * 8;
+        const float4 a = q[0], b = q[1], d = q[2], e = q[3], g4 = q[4], h = q[5], r4 = q[6], m = q[7];
+        Frame& f = F[0];
+        f.depth = __float_as_int(a.w);
+        f.type = __float_as_int(b.w);
+        f.albedo = f3(d.x, d.y, d.z);
+        f.w0 = d.w;
+        f.ro = f3(e.x, e.y, e.z);
+        f.w1 = e.w;
+        f.rd = f3(g4.x, g4.y, g4.z);
+        Ray rr;
+        rr.o = f3(a.x, a.y, a.z);
+        rr.d = f3(b.x, b.y, b.z);
+        Rng rng(1u);
+        const f3 gv = whitted(S, st, rr, f.depth - 1, rng, F, 1, stk, c);
+        const f3 T(h.x, h.y, h.z), E(m.x, m.y, m.z);
+        f3 v(r4.x, r4.y, r4.z);
+        v += T * gv - T * E;   // P.R += P.T * g - P.T * emission
+        const int64_t p = __float_as_int(g4.w), k = __float_as_int(h.w);
+        float* o = samp + (k * n_px + p) * 3;
+        o[0] = v.x; o[1] = v.y; o[2] = v.z;
+    }
+    flush_counts(c, ctr);
+}
+
+// Pixels that deferred a sample: the prefix plus samples first..n^2-1 in
+// order, divided by n^2 -- render_pixel's sum, same additions, same order.
+__global__ __launch_bounds__(256) void k_sum(float* __restrict__ out, const float* __restrict__ samp,
+                                             const float* __restrict__ prefix, const int32_t* __restrict__ first,
+                                             int64_t n_px, int nn) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_px) return;
+    const int k0 = first[p];
+    if (k0 >= nn) return;
+    f3 acc(prefix[p * 3], prefix[p * 3 + 1], prefix[p * 3 + 2]);
+    for (int k = k0; k < nn; ++k) {
+        const float* s = samp + ((int64_t)k * n_px + p) * 3;
+        acc += f3(s[0], s[1], s[2]);
+    }
+    const f3 px = acc / (float)nn;
+    out[p * 3] = px.x; out[p * 3 + 1] = px.y; out[p * 3 + 2] = px.z;
 }
 
 struct SceneSlot {
@@ -250,6 +344,7 @@ struct SceneSlot {
     std::vector<int32_t> leaf_order;
     std::vector<LightRec> lights;   // host copy, for per-item sample budgets
     int32_t n_mats = 0;
+    bool has_glass = false;         // some material is a dielectric
     pcl_scene_info info{};
 };
 
@@ -282,6 +377,20 @@ struct pcl_ctx {
     // per 128 pixels, 1 persistent.  Environment PCL_RENDER_KERNEL overrides.
     int render_kernel = -1;
     int render_bpc[3] = {0, 0, 0};        // persistent blocks per CU (Whitted, path, depth <= 1); 0 = occupancy
+    // deferred glass vertices of path-traced renders (render_pixel_defer),
+    // grown on demand: tasks, per-sample values, per-pixel prefix / first
+    float4* d_gtasks = nullptr;
+    size_t gtask_cap = 0;
+    uint32_t* d_gcount = nullptr;         // one 128-byte line
+    float* d_samp = nullptr;
+    size_t samp_cap = 0;                  // floats
+    float* d_prefix = nullptr;
+    int32_t* d_first = nullptr;
+    size_t px_cap = 0;
+    // -1 automatic (defer when the scene allows it), 0 never, 1 whenever
+    // allowed.  Environment PCL_DEFER_GLASS overrides.
+    int defer_glass = -1;
+    int64_t defer_tasks = 0;              // task array size override (PCL_DEFER_TASKS); 0 = automatic
 };
 
 namespace {
@@ -358,6 +467,7 @@ void fill_stats(pcl_stats* st, const unsigned long long* h, float kms, double t0
     st->anyhit_queries = h[CTR_ANY];
     st->kernel_ms = kms;
     st->total_ms = now_ms() - t0;
+    st->deferred_glass = 0;
 }
 
 }  // namespace
@@ -393,6 +503,9 @@ int pcl_init(int device, pcl_ctx** out) try {
     }
     if (hipMalloc(&ctx->d_heads, kRHeadWords * sizeof(uint32_t)) != hipSuccess) return PCL_ERR_OOM;
     if (const char* e = std::getenv("PCL_RENDER_KERNEL")) ctx->render_kernel = std::atoi(e);
+    if (const char* e = std::getenv("PCL_DEFER_GLASS")) ctx->defer_glass = std::atoi(e);
+    if (const char* e = std::getenv("PCL_DEFER_TASKS")) ctx->defer_tasks = std::max<int64_t>(1, std::atoll(e));
+    if (hipMalloc(&ctx->d_gcount, 128) != hipSuccess) return PCL_ERR_OOM;
     if (const char* e = std::getenv("PCL_RENDER_BPC"))
         ctx->render_bpc[0] = ctx->render_bpc[1] = ctx->render_bpc[2] = std::atoi(e);
     // the device address of this device's scratch array (device is current)
@@ -420,6 +533,9 @@ void pcl_destroy(pcl_ctx* ctx) {
     if (ctx->d_thr) (void)hipFree(ctx->d_thr);
     if (ctx->d_rgb8) (void)hipFree(ctx->d_rgb8);
     if (ctx->d_heads) (void)hipFree(ctx->d_heads);
+    for (void* p : {(void*)ctx->d_gtasks, (void*)ctx->d_gcount, (void*)ctx->d_samp, (void*)ctx->d_prefix,
+                    (void*)ctx->d_first})
+        if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -558,6 +674,7 @@ int pcl_scene_upload(pcl_ctx* ctx, const pcl_scene_desc* desc, int32_t* scene_ou
     sl.leaf_order = std::move(L.leaf_order);
     sl.lights = L.lights;
     sl.n_mats = desc->n_materials;
+    for (int32_t m = 0; m < desc->n_materials; ++m) sl.has_glass |= desc->materials[m].kind == PCL_MAT_DIELECTRIC;
     sl.info.n_prims = desc->n_prims;
     sl.info.n_nodes = device_build ? desc->n_prims - 1 : (int32_t)L.nodes.size();
     sl.info.depth = L.depth;
@@ -701,6 +818,48 @@ int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, in
     return PCL_OK;
 } PCL_CATCH(ctx)
 
+// Deferred glass vertices (render_pixel_defer): path tracing, a scene with a
+// dielectric and no point light with a radius (Whitted then draws no random
+// numbers), and buffers that fit: the per-sample values of the frame (12 B
+// per pixel and sample) and a task array sized for an eighth of the samples
+// (128 B each; tasks beyond it are evaluated in place), together under half
+// of the device's free memory.  Returns false (render in place) otherwise.
+static bool defer_buffers(pcl_ctx* ctx, const SceneSlot& sl, const RSettings& st) {
+    if (!st.pathtrace || !sl.has_glass || ctx->defer_glass == 0) return false;
+    for (const LightRec& L : sl.lights)
+        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) return false;
+    const int64_t n_px = (int64_t)st.width * st.height, nn = (int64_t)st.samples * st.samples;
+    const size_t samp = (size_t)(n_px * nn * 3);
+    const size_t tasks = ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks
+                                              : (size_t)std::min<int64_t>(n_px * nn / 8 + 4096, (int64_t)1 << 27);
+    size_t need = 0;
+    if (samp > ctx->samp_cap) need += samp * sizeof(float);
+    if (tasks > ctx->gtask_cap) need += tasks * 8 * sizeof(float4);
+    if ((size_t)n_px > ctx->px_cap) need += (size_t)n_px * 16;
+    if (need) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 2) return false;
+    }
+    auto grow = [](auto** p, size_t& cap, size_t want, size_t elem) -> bool {
+        if (want <= cap) return true;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        cap = 0;
+        if (hipMalloc((void**)p, want * elem) != hipSuccess) { *p = nullptr; return false; }
+        cap = want;
+        return true;
+    };
+    size_t pxc = ctx->px_cap;
+    if (!grow(&ctx->d_samp, ctx->samp_cap, samp, sizeof(float)) ||
+        !grow(&ctx->d_gtasks, ctx->gtask_cap, tasks, 8 * sizeof(float4)) ||
+        !grow(&ctx->d_prefix, pxc, (size_t)n_px, 3 * sizeof(float)))
+        return false;
+    size_t pxc2 = ctx->px_cap;
+    if (!grow(&ctx->d_first, pxc2, (size_t)n_px, sizeof(int32_t))) return false;
+    ctx->px_cap = std::min(pxc, pxc2);
+    return true;
+}
+
 static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                        float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) {
     SceneSlot* sl = get_scene(ctx, scene);
@@ -717,9 +876,22 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     const int n_chunks = ((st.width + 63) / 64) * st.height;
     // Whitted with max_depth <= 1 has no recursion: the frame-free program
     const int mode = st.pathtrace ? 1 : (st.max_depth <= 1 ? 2 : 0);
-    auto pkern = mode == 1 ? (counts ? k_render_p<true, 1> : k_render_p<false, 1>)
-               : mode == 2 ? (counts ? k_render_p<true, 2> : k_render_p<false, 2>)
-                           : (counts ? k_render_p<true, 0> : k_render_p<false, 0>);
+    const bool defer = mode == 1 && defer_buffers(ctx, *sl, st);
+    DeferBufs db{};
+    if (defer) {
+        db.sink.tasks = ctx->d_gtasks;
+        db.sink.count = ctx->d_gcount;
+        db.sink.cap = (uint32_t)std::min<size_t>(ctx->gtask_cap, ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks
+                                                                                       : ctx->gtask_cap);
+        db.samp = ctx->d_samp;
+        db.prefix = ctx->d_prefix;
+        db.first = ctx->d_first;
+        db.n_px = (int64_t)st.width * st.height;
+    }
+    auto pkern = mode == 1 ? (defer ? (counts ? k_render_p<true, 1, true> : k_render_p<false, 1, true>)
+                                    : (counts ? k_render_p<true, 1, false> : k_render_p<false, 1, false>))
+               : mode == 2 ? (counts ? k_render_p<true, 2, false> : k_render_p<false, 2, false>)
+                           : (counts ? k_render_p<true, 0, false> : k_render_p<false, 0, false>);
     int& bpc = ctx->render_bpc[mode];
     if (bpc <= 0) {
         int nb = 0;
@@ -733,23 +905,44 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     hipLaunchKernelGGL(k_zero2_u64, dim3(kRHeadWords / 2 / 64), dim3(64), 0, ctx->stream, ctx->d_ctr, kCtrSlots,
                        reinterpret_cast<unsigned long long*>(ctx->d_heads), persist ? kRHeadWords / 2 : 0);
     HIP_TRY(ctx, hipGetLastError());
+    if (defer) {   // the task count (its own line)
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(16), 0, ctx->stream, reinterpret_cast<unsigned long long*>(ctx->d_gcount),
+                           16, 0ull);
+        HIP_TRY(ctx, hipGetLastError());
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (persist) {
         hipLaunchKernelGGL(pkern, dim3(pgrid), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr,
-                           ctx->d_heads, n_chunks);
+                           ctx->d_heads, n_chunks, db);
     } else {
-        auto kern = mode == 1 ? (counts ? k_render<true, 1> : k_render<false, 1>)
-                  : mode == 2 ? (counts ? k_render<true, 2> : k_render<false, 2>)
-                              : (counts ? k_render<true, 0> : k_render<false, 0>);
-        hipLaunchKernelGGL(kern, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr);
+        auto kern = mode == 1 ? (defer ? (counts ? k_render<true, 1, true> : k_render<false, 1, true>)
+                                       : (counts ? k_render<true, 1, false> : k_render<false, 1, false>))
+                  : mode == 2 ? (counts ? k_render<true, 2, false> : k_render<false, 2, false>)
+                              : (counts ? k_render<true, 0, false> : k_render<false, 0, false>);
+        hipLaunchKernelGGL(kern, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr, db);
     }
     HIP_TRY(ctx, hipGetLastError());
+    if (defer) {
+        int nb = 0;
+        HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, counts ? k_glass<true> : k_glass<false>, kBlock, 0));
+        const int ggrid = ctx->cus * std::max(nb, 1);
+        hipLaunchKernelGGL(counts ? k_glass<true> : k_glass<false>, dim3(ggrid), dim3(kBlock), 0, ctx->stream, sl->dev,
+                           st, ctx->d_gtasks, ctx->d_gcount, db.sink.cap, ctx->d_samp, db.n_px, ctx->d_ctr);
+        HIP_TRY(ctx, hipGetLastError());
+        hipLaunchKernelGGL(k_sum, dim3((unsigned)((db.n_px + 255) / 256)), dim3(256), 0, ctx->stream, d_out_rgb,
+                           ctx->d_samp, ctx->d_prefix, ctx->d_first, db.n_px, st.samples * st.samples);
+        HIP_TRY(ctx, hipGetLastError());
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t n_deferred = 0;
+    if (defer)
+        HIP_TRY(ctx, hipMemcpyAsync(&n_deferred, ctx->d_gcount, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     float kms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
     fill_stats(stats, h, kms, t0);
+    if (stats) stats->deferred_glass = std::min<uint64_t>(n_deferred, db.sink.cap);
     fill_counts(counts, h);
     return PCL_OK;
 }
@@ -993,6 +1186,8 @@ int pcl_scene_set_materials(pcl_ctx* ctx, int32_t scene, const pcl_material* mat
     const int rc = pack_materials(materials, n, recs, err);
     if (rc) return fail(ctx, rc, err);
     if (n == 0) return PCL_OK;
+    sl->has_glass = false;
+    for (int32_t m = 0; m < n; ++m) sl->has_glass |= materials[m].kind == PCL_MAT_DIELECTRIC;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipMemcpyAsync(const_cast<MatRec*>(sl->dev.mats), recs.data(), recs.size() * sizeof(MatRec),
                                 hipMemcpyHostToDevice, ctx->stream));

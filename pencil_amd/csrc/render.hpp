@@ -218,6 +218,44 @@ PCL_D Ray push_glass(FR* F, int& nf, const Ray& r, const HitRec& rec, const f3& 
     return rr;
 }
 
+// Deferred glass vertices.  A path that hits glass ends there: its sample
+// is R + (T * g - T * emission), g the value of trace_dielectric's Whitted
+// subtree (renderer.hpp:941-949).  When no point light has a radius, Whitted
+// draws no random numbers, so g depends on nothing the path's later samples
+// use: the per-pixel program can record the glass vertex as a task and go on
+// with its next sample, and a separate launch (core.hip: k_glass) evaluates
+// the subtrees of all tasks with full waves instead of one lane while its
+// wave waits.  A task is 8 float4: the reflection ray, the glass frame
+// (push_glass), T, R, emission, and the sample it completes.
+struct GlassSink {
+    float4* tasks;       // 8 float4 per task
+    uint32_t* count;     // tasks appended; may pass cap (those were not deferred)
+    uint32_t cap;
+    int32_t pixel, k;    // the sample being traced
+    // Wave-aggregated append (every lane of the branch calls it); false when
+    // the task array is full and the caller evaluates the vertex itself.
+    PCL_D bool append(const Frame& f, const Ray& rr, const f3& T, const f3& R, const f3& e) {
+        const unsigned long long m = __ballot(1);
+        const int lane = threadIdx.x & 63;
+        const int leader = __ffsll((long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+        base = __shfl(base, leader, 64);
+        const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (idx >= cap) return false;
+        float4* q = tasks + (size_t)idx * 8;
+        q[0] = make_float4(rr.o.x, rr.o.y, rr.o.z, __int_as_float(f.depth));
+        q[1] = make_float4(rr.d.x, rr.d.y, rr.d.z, __int_as_float(f.type));
+        q[2] = make_float4(f.albedo.x, f.albedo.y, f.albedo.z, f.w0);
+        q[3] = make_float4(f.ro.x, f.ro.y, f.ro.z, f.w1);
+        q[4] = make_float4(f.rd.x, f.rd.y, f.rd.z, __int_as_float(pixel));
+        q[5] = make_float4(T.x, T.y, T.z, __int_as_float(k));
+        q[6] = make_float4(R.x, R.y, R.z, 0.0f);
+        q[7] = make_float4(e.x, e.y, e.z, 0.0f);
+        return true;
+    }
+};
+
 // State of one Renderer::trace recursion (renderer.hpp:403-822, in-core
 // branches) between trace() calls: the ray and depth of the next call and the
 // frames of the calls waiting for their children.
@@ -366,6 +404,7 @@ struct PathState {
     Ray cur;
     f3 T, R;
     int depth;
+    bool deferred = false;   // the walk ended at a deferred glass vertex
 };
 
 // One iteration of trace_path's bounce loop (its caller checks depth > 0):
@@ -374,8 +413,15 @@ struct PathState {
 // subtree runs here as trace_dielectric does).  With st.pathtrace == 2,
 // trace_path_mis (renderer.hpp:1205-1335), which differs only in the light
 // loop (its tangent frame is the diffuse branch's, computed earlier).
-template <class LC>
-PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& rng, Frame* F, Stack& stk, LC& c) {
+struct NoSink {
+    PCL_D bool append(const Frame&, const Ray&, const f3&, const f3&, const f3&) { return false; }
+};
+
+// `sink`: where a glass vertex may be deferred (GlassSink); P.deferred is
+// then set and P.R is not the sample's value.
+template <class LC, class SINK = NoSink>
+PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& rng, Frame* F, Stack& stk, LC& c,
+                     SINK* sink = nullptr) {
     const bool mis = st.pathtrace == 2;
     const int bounces = st.bounces;
     HitRec rec;
@@ -423,6 +469,10 @@ PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& 
     } else if (kind == PCL_MAT_DIELECTRIC) {
         int nf = 0;
         Ray rr = push_glass(F, nf, P.cur, rec, albedo, ior, P.depth, st.shadow_bias);
+        if (sink && sink->append(F[0], rr, P.T, P.R, emission)) {
+            P.deferred = true;
+            return false;
+        }
         f3 g = whitted(S, st, rr, P.depth - 1, rng, F, nf, stk, c);
         P.R += P.T * g - P.T * emission;
         return false;
@@ -478,6 +528,48 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
         }
     }
     return acc / (float)(n * n);
+}
+
+// Glass subtrees evaluated by a later launch (trace_path with a GlassSink).
+// render_pixel's loop, except that once a sample is deferred the sum stops:
+// acc then holds the sum of the samples before the first deferred one
+// (`first`), and every later sample evaluated here is stored in samp (sample
+// k of this pixel at samp[k * stride]), the deferred ones being filled in by
+// the glass launch; a last launch adds them to acc in sample order, so the
+// pixel gets render_pixel's additions in render_pixel's order.  Returns
+// true, with the pixel value in px, when no sample was deferred.
+template <class LC>
+PCL_D bool render_pixel_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F,
+                              Stack& stk, LC& c, GlassSink& sink, float* samp, size_t stride, f3& px, int& first) {
+    const int W = st.width, H = st.height, n = st.samples;
+    Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
+    f3 acc(0, 0, 0);
+    first = n * n;
+    for (int k = 0; k < n * n; ++k) {
+        const int sy = k / n, sx = k - sy * n;
+        float u = (i + (sx + 0.5f) / n) / (float)W;
+        float v = 1.0f - (j + (sy + 0.5f) / n) / (float)H;
+        u += rng.sym() * (0.5f / (W * n));
+        v += rng.sym() * (0.5f / (H * n));
+        Ray r = camera_ray(C, u, v, rng);
+        c.rays++;   // trace_path()
+        PathState P;
+        P.cur = r; P.T = f3(1, 1, 1); P.R = f3(0, 0, 0); P.depth = st.bounces;
+        sink.k = k;
+        while (P.depth > 0) {
+            if (!path_step(S, st, P, rng, F, stk, c, &sink)) break;
+        }
+        if (P.deferred) {
+            if (first == n * n) first = k;
+        } else if (first == n * n) {
+            acc += P.R;
+        } else {
+            float* o = samp + (size_t)k * stride;
+            o[0] = P.R.x; o[1] = P.R.y; o[2] = P.R.z;
+        }
+    }
+    px = (first == n * n) ? acc / (float)(n * n) : acc;
+    return first == n * n;
 }
 
 // render_pixel from sample k0 on: `rng` is the pixel's stream as sample k0

@@ -1,0 +1,126 @@
+"""Deferred glass vertices in the path-traced megakernel.
+
+In a scene with a dielectric and no point light with a radius, a path that
+hits glass records the vertex as a task and goes on with its next sample; a
+second launch evaluates the Whitted subtrees of all tasks and a third adds
+each deferred pixel's samples in sample order.  The image and every count
+must be the ones of the in-place program bit for bit: the same samples, the
+same additions in the same order.  The task array can fill up, in which case
+the remaining glass vertices are evaluated in place, also bit for bit.
+
+The scenes' point lights get radius 0 here (the parser's default is 0.5,
+which makes Whitted draw random numbers and rules deferral out); the
+deferred_glass count shows that the deferred program ran.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pencil_amd as pa
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+CORNELL = os.path.join(os.path.dirname(pa.__file__), "scenes", "cornell_c5.pecl")
+
+
+def _core(**env):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return pa.Core(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.fixture(scope="module")
+def cores():
+    c_off, c_on, c_small = _core(PCL_DEFER_GLASS=0), _core(PCL_DEFER_GLASS=1), \
+        _core(PCL_DEFER_GLASS=1, PCL_DEFER_TASKS=37)
+    yield c_off, c_on, c_small
+    for c in (c_off, c_on, c_small):
+        c.close()
+
+
+def _hard(hs):
+    """Point lights of the host scene set to radius 0 (in place)."""
+    d = hs.desc()
+    for i in range(d.n_lights):
+        d.lights[i].radius = 0.0
+    return hs
+
+
+def _scene(name, w, h, side, pathtrace, bounces):
+    path = CORNELL if name == "cornell_c5" else golden(name + ".pecl")
+    hs = _hard(pa.HostScene.from_pecl(path, width=w, height=h, samples=side))
+    st = hs.settings()
+    st.pathtrace, st.pathtrace_bounces = pathtrace, bounces
+    return hs, st
+
+
+def _render(core, hs, st):
+    sid = core.upload(hs.desc())
+    try:
+        return core.render(sid, hs.camera(), st)
+    finally:
+        core.free_scene(sid)
+
+
+def _same(a, sa, b, sb):
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (sa.rays_traced, sa.closest_queries, sa.anyhit_queries) == \
+        (sb.rays_traced, sb.closest_queries, sb.anyhit_queries)
+
+
+CASES = [("cornell_c5", 96, 72, 4, 1, 8), ("cornell_c5", 96, 72, 4, 2, 8), ("cornell_c5", 80, 60, 3, 1, 3),
+         ("three_spheres", 64, 48, 3, 1, 8), ("glass", 64, 48, 2, 1, 6), ("glass", 64, 48, 2, 2, 5)]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}x{c[2]}-{c[3]}x{c[3]}spp-pt{c[4]}-b{c[5]}" for c in CASES])
+def test_deferred_glass_is_bit_identical(cores, case):
+    c_off, c_on, c_small = cores
+    hs, st = _scene(*case)
+    ref, sr = _render(c_off, hs, st)
+    assert sr.deferred_glass == 0
+    img, s = _render(c_on, hs, st)
+    _same(img, s, ref, sr)
+    assert s.deferred_glass > 37
+    img, s = _render(c_small, hs, st)   # a 37-task array: the rest in place
+    _same(img, s, ref, sr)
+    assert s.deferred_glass == 37
+
+
+def test_deferred_glass_on_the_1m_triangle_cornell_frame(cores):
+    """The path-traced Cornell + mesh workload's scene at 320x180, 4 spp,
+    8 bounces with Russian roulette: both grids' sizes of image reach the
+    deferred program (a persistent grid needs a large image)."""
+    from pencil_amd import workloads as wl
+    c_off, c_on, _ = cores
+    hs, st = wl.cornell_pt(pa, width=320, height=180, samples=2)
+    _hard(hs)
+    ref, sr = _render(c_off, hs, st)
+    img, s = _render(c_on, hs, st)
+    _same(img, s, ref, sr)
+    assert s.deferred_glass > 0
+
+
+def test_soft_lights_render_in_place(cores):
+    """A point light with a radius makes Whitted draw random numbers (the
+    light jitter), so such scenes are never deferred; the images agree."""
+    c_off, c_on, _ = cores
+    hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=48, height=36, samples=2)
+    st = hs.settings()
+    st.pathtrace, st.pathtrace_bounces = 1, 5
+    d = hs.desc()
+    assert d.n_lights >= 2
+    d.lights[0].radius = 0.0
+    d.lights[1].radius = 0.3   # one soft light is enough
+    ref, sr = _render(c_off, hs, st)
+    img, s = _render(c_on, hs, st)
+    _same(img, s, ref, sr)
+    assert s.deferred_glass == 0
