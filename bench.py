@@ -308,6 +308,49 @@ def cpu_stress(queries, threads, model):
 
 
 # ---------------------------------------------------------------------------
+# glass_defer: the Cornell + mesh frame with a hard point light (radius 0),
+# plain renders with the glass vertices evaluated in place and deferred
+# (not a configured workload: the configured scene's light has the parser's
+# default radius 0.5, whose jitter rules deferral out)
+# ---------------------------------------------------------------------------
+
+def leg_glass_defer(core, pa, args, threads, model):
+    hs, st = wl.cornell_pt(pa, samples=args.pt_samples)
+    desc, cam = hs.desc(), hs.camera()
+    for i in range(desc.n_lights):
+        desc.lights[i].radius = 0.0
+    W, H = st.width, st.height
+    out, imgs = {}, {}
+    for name, env in (("in_place", "0"), ("deferred", "1")):
+        os.environ["PCL_DEFER_GLASS"] = env
+        c = pa.Core(0)
+        try:
+            sid = c.upload(desc)
+            d = c.alloc(W * H * 12)
+            c.render_device(sid, cam, st, d)   # warm-up (buffers)
+            t0 = time.perf_counter()
+            s = c.render_device(sid, cam, st, d)
+            ms = (time.perf_counter() - t0) * 1000.0
+            img = np.empty((H, W, 3), dtype=np.float32)
+            c.d2h(img, d)
+            imgs[name] = img
+            q = s.closest_queries + s.anyhit_queries
+            out[name] = {"ms": ms, "kernel_ms": s.kernel_ms, "mrays_s": q / ms / 1e3, "queries": q,
+                         "deferred_glass_vertices": s.deferred_glass}
+            log(f"glass_defer {name}: {ms:.1f} ms, {q / ms / 1e3:.1f} Mrays/s, {s.deferred_glass} deferred")
+            c.free(d)
+            c.free_scene(sid)
+        finally:
+            c.close()
+    os.environ.pop("PCL_DEFER_GLASS", None)
+    out["workload"] = (f"Cornell box + {desc.n_prims - 14}-tri mesh with the point light's radius set to 0, {W}x{H}, "
+                       f"{st.samples ** 2} spp, path tracing, 8 bounces, Russian roulette after 3 (plain renders)")
+    out["speedup_deferred"] = out["in_place"]["ms"] / out["deferred"]["ms"]
+    out["bit_identical"] = bool(np.array_equal(imgs["in_place"].view(np.uint32), imgs["deferred"].view(np.uint32)))
+    return rounded(out)
+
+
+# ---------------------------------------------------------------------------
 # cornell_pt: path tracing, plain vs batched
 # ---------------------------------------------------------------------------
 
@@ -665,7 +708,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--plain-only", action="store_true", help="skip the batched leg of the headline workload")
     ap.add_argument("--legs", default="cornell_pt,batch64,inverse",
-                    help="further workloads to run after the headline one (comma list, or 'none')")
+                    help="further workloads to run after the headline one (comma list, or 'none'); "
+                         "glass_defer (not run by default) measures deferred glass vertices on a hard-light frame")
     ap.add_argument("--pt-samples", type=int, default=wl.CORNELL_PT["samples"],
                     help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",
@@ -699,7 +743,8 @@ def main():
     r = None if args.skip_headline else leg_stress(core, pa, args)
     legs = {}
     for name in [x for x in args.legs.split(",") if x and x != "none"]:
-        fn = {"cornell_pt": leg_cornell_pt, "batch64": leg_batch64, "inverse": leg_inverse}[name]
+        fn = {"cornell_pt": leg_cornell_pt, "batch64": leg_batch64, "inverse": leg_inverse,
+              "glass_defer": leg_glass_defer}[name]
         t0 = time.time()
         legs[name] = fn(core, pa, args, threads, model)
         log(f"leg {name}: {time.time() - t0:.1f} s")
