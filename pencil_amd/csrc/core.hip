@@ -104,11 +104,6 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
 template <int PT>
 struct RenderWaves { static constexpr int v = PT == 1 ? 4 : PT == 2 ? PCL_SHALLOW_WAVES : 3; };
 
-// One instantiation per tracing mode (PT: Whitted / path), each with the
-// register budget measured fastest for it: Whitted at most 168 VGPRs, 3 waves
-// per SIMD (faster than 183 / 2 waves; 128 / 4 waves is no faster); path
-// tracing 128 VGPRs, 4 waves (the 1M-triangle Cornell frame 1.19x faster
-// than at 168 / 3 waves, and faster than 96 / 5 waves).
 // Buffers of the deferred glass vertices (render_pixel_defer, k_glass, k_sum).
 struct DeferBufs {
     GlassSink sink;    // tasks, count, cap
@@ -144,6 +139,11 @@ __device__ __forceinline__ void shade_pixel(const DevScene& S, const CamRec& C, 
     o[0] = px.x; o[1] = px.y; o[2] = px.z;
 }
 
+// One instantiation per tracing mode (PT: Whitted / path), each with the
+// register budget measured fastest for it: Whitted at most 168 VGPRs, 3 waves
+// per SIMD (faster than 183 / 2 waves; 128 / 4 waves is no faster); path
+// tracing 128 VGPRs, 4 waves (the 1M-triangle Cornell frame 1.19x faster
+// than at 168 / 3 waves, and faster than 96 / 5 waves).
 template <bool COUNT, int PT, bool DEFER>
 __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
                                                    unsigned long long* ctr, DeferBufs db) {
@@ -829,6 +829,7 @@ static bool defer_buffers(pcl_ctx* ctx, const SceneSlot& sl, const RSettings& st
     for (const LightRec& L : sl.lights)
         if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) return false;
     const int64_t n_px = (int64_t)st.width * st.height, nn = (int64_t)st.samples * st.samples;
+    if (n_px * nn >= ((int64_t)1 << 32)) return false;   // the task counter is 32-bit
     const size_t samp = (size_t)(n_px * nn * 3);
     const size_t tasks = ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks
                                               : (size_t)std::min<int64_t>(n_px * nn / 8 + 4096, (int64_t)1 << 27);
