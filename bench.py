@@ -138,7 +138,59 @@ def profile_traffic(name):
     return int(round(t["hbm_bytes_per_dispatch"])), t
 
 
-def roofline(bytes_per_launch, kernel_ms, traffic_name, scope, limiter):
+def _pmc_run(counter, args, tmp, name):
+    """One rocprofv3 PMC pass (one counter) over `python3 bench.py <args>` in a child
+    process; returns the rows of its counter CSV."""
+    import csv
+    out = os.path.join(tmp, name)
+    cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "run", "--",
+           sys.executable, os.path.join(ROOT, "bench.py")] + args
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=240)
+    files = [os.path.join(dp, f) for dp, _, fs in os.walk(out) for f in fs if f.endswith("counter_collection.csv")]
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter}: no counter file (rc {r.returncode}): {r.stderr[-300:]}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def live_traffic(batched_cfg):
+    """HBM bytes of the headline frame measured in this run: FETCH_SIZE and WRITE_SIZE
+    of the plain kernel's two timed frames and of the batched frames (every wavefront
+    kernel, per frame), from child runs of this script under rocprofv3, each counter
+    scaled by what it reads for a 256 MiB copy kernel.  Returns (plain, batched)
+    dicts, or (None, None) when a pass fails (the committed profiles are used then)."""
+    tmp = tempfile.mkdtemp(prefix="pcl_pmc_", dir="/tmp")
+    try:
+        args = ["--no-cpu", "--no-live-counters", "--legs", "none", "--steps", "2", "--warmup", "0",
+                "--batched-config", batched_cfg]
+        scale = {}
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows = _pmc_run(ctr, ["--copy-probe"], tmp, "copy_" + ctr)
+            v = [float(r["Counter_Value"]) for r in rows if "AUnaryFunctor" in r["Kernel_Name"]]
+            scale[ctr] = 262144.0 / (sum(v) / len(v))   # 256 MiB per dispatch, counters in KiB
+        got = {"plain": {}, "batched": {}}
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            rows = _pmc_run(ctr, args, tmp, "head_" + ctr)
+            plain = sorted({int(r["Dispatch_Id"]): float(r["Counter_Value"]) for r in rows
+                            if "k_render_p<false, 2, false>" in r["Kernel_Name"]}.items())[:2]
+            wf = [float(r["Counter_Value"]) for r in rows if "pcl::wf::" in r["Kernel_Name"]]
+            if len(plain) != 2 or not wf:
+                raise RuntimeError(f"unexpected dispatches under rocprofv3 ({len(plain)} plain)")
+            got["plain"][ctr] = sum(v for _, v in plain) / 2 * 1024 * scale[ctr]
+            got["batched"][ctr] = sum(wf) / 2 * 1024 * scale[ctr]
+        res = tuple({"read_bytes": g["FETCH_SIZE"], "write_bytes": g["WRITE_SIZE"],
+                     "hbm_bytes": g["FETCH_SIZE"] + g["WRITE_SIZE"], "fetch_scale": scale["FETCH_SIZE"],
+                     "write_scale": scale["WRITE_SIZE"]} for g in (got["plain"], got["batched"]))
+        return res
+    except Exception as e:   # no profiler, a failed pass: fall back to the committed profiles
+        log(f"live HBM counters not measured ({e}); the rooflines keep the committed profiles' traffic")
+        return None, None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def roofline(bytes_per_launch, kernel_ms, traffic_name, scope, limiter, live=None):
     """Algorithmic-byte roofline of a kernel (or a job of kernels), with the
     counter-measured HBM traffic, L2 hit rate and VALU issue beside it.
     `frac` is algorithmic-byte throughput over the HBM peak; `limiter` says
@@ -146,8 +198,11 @@ def roofline(bytes_per_launch, kernel_ms, traffic_name, scope, limiter):
     bandwidth-bound: the BVH sits in the 256 MB MALL)."""
     achieved = bytes_per_launch / (kernel_ms / 1000.0) / 1e9
     traffic, t = profile_traffic(traffic_name)
+    source = f"profiles/{traffic_name}.traffic.json"
+    if live:   # counters of this run
+        traffic, source = int(round(live["hbm_bytes"])), "rocprofv3 FETCH_SIZE / WRITE_SIZE passes in this run"
     r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "scope": scope,
+         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": source, "scope": scope,
          "algorithmic_bytes": int(bytes_per_launch), "limiter": limiter}
     if traffic:
         # the rate the counters see, over the kernel time measured here
@@ -719,6 +774,9 @@ def main():
                     help="1: the inverse leg also reports the Gauss-Newton conditioning at the true parameters")
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the legs (profiling runs); prints the legs' JSON, not the bench line")
+    ap.add_argument("--no-live-counters", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes that measure the headline frame's HBM "
+                         "traffic in this run (the rooflines then keep the committed profiles' traffic)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 256 MiB read + 256 MiB write elementwise kernel (rocprof PMC calibration)")
     args = ap.parse_args()
@@ -762,6 +820,20 @@ def main():
             log(f"cpu baseline: {HARNESS} not built (make -C oracle ref); reported as missing")
 
     value, cst, info, batched = r["value"], r["cst"], r["info"], r["batched"]
+    live_plain = live_batched = None
+    if not args.no_live_counters:
+        t0 = time.time()
+        bc = batched["config"] if batched else {"slots": 0, "blocks_per_cu": 0, "stack_k": 0, "iters_per_graph": 0,
+                                                 "drain_below": 0, "inline_shadows": 0, "streams": 0}
+        cfg = ",".join(str(bc[k]) for k in ("slots", "blocks_per_cu", "stack_k", "iters_per_graph", "drain_below",
+                                            "inline_shadows", "streams"))
+        live_plain, live_batched = live_traffic(cfg)
+        if live_plain:
+            log(f"live HBM counters: plain frame {live_plain['hbm_bytes'] / 1e6:.1f} MB, batched frame "
+                f"{live_batched['hbm_bytes'] / 1e6:.1f} MB ({time.time() - t0:.1f} s)")
+    if batched and live_batched:
+        batched["roofline"] = rounded(roofline(r["bytes"], batched["kernel_ms"], "wavefront_bvh_stress_1080p", "job",
+                                               batched["roofline"]["limiter"], live=live_batched))
     line = {
         "metric": "Mrays/s at 1080p 1M-tri (plain vs batched); adjoint grad-steps/s at FD tol",
         "value": round(value, 3),
@@ -788,7 +860,7 @@ def main():
                    "adjoint_grad_steps_per_s": (round(legs["inverse"]["grad_steps_per_s"], 3)
                                                 if "inverse" in legs else None)},
         "roofline": roofline(r["bytes"], r["kernel_ms"], "k_render_bvh_stress_1080p", "kernel",
-                                "dependent node-fetch latency of the traversal"),
+                             "dependent node-fetch latency of the traversal", live=live_plain),
         "cpu_baseline": cpu,
         "legs": legs,
     }
