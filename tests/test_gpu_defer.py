@@ -124,3 +124,22 @@ def test_soft_lights_render_in_place(cores):
     img, s = _render(c_on, hs, st)
     _same(img, s, ref, sr)
     assert s.deferred_glass == 0
+
+
+def test_deferred_glass_persistent_grid_and_many_tasks():
+    """The persistent grid (forced) with deferral, on enough samples that the
+    glass launch strides over its grid more than once (> 100 k deferred
+    vertices): the in-place persistent render's image and counts."""
+    from pencil_amd import workloads as wl
+    c_off = _core(PCL_DEFER_GLASS=0, PCL_RENDER_KERNEL=1)
+    c_on = _core(PCL_DEFER_GLASS=1, PCL_RENDER_KERNEL=1)
+    try:
+        hs, st = wl.cornell_pt(pa, width=640, height=360, samples=4)
+        _hard(hs)
+        ref, sr = _render(c_off, hs, st)
+        img, s = _render(c_on, hs, st)
+    finally:
+        c_off.close()
+        c_on.close()
+    _same(img, s, ref, sr)
+    assert s.deferred_glass > 100_000
