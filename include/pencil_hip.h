@@ -142,7 +142,7 @@ typedef struct pcl_stats {
     double   kernel_ms;         /* HIP-event time of the launches on the context stream */
     double   total_ms;          /* host wall time of the call */
     uint64_t deferred_glass;    /* path-traced renders: glass vertices whose Whitted subtree a second
-                                   launch evaluated (scenes without point-light radii; 0 otherwise) */
+                                   launch evaluated (0 when none was deferred) */
 } pcl_stats;
 
 /* Ray batches for the traversal entry point. */

@@ -273,11 +273,10 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScen
 }
 
 // The deferred glass vertices: one lane per task (grid-stride over 64-task
-// chunks), trace_dielectric's Whitted subtree from the recorded frame --
-// whitted() with the frame pushed, as path_step runs it -- then the sample's
-// value R + (T * g - T * emission) into its slot.  Whitted draws no random
-// numbers in a scene without point-light radii (the only scenes deferred),
-// so the generator passed here is never advanced.
+// chunks), trace_dielectric's Whitted subtree from the recorded frame and
+// generator state -- whitted() with the frame pushed, as path_step runs it,
+// so the same draws -- then the sample's value R + (T * g - T * emission)
+// into its slot.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock, 3) void k_glass(DevScene S, RSettings st, const float4* __restrict__ tasks,
                                                      const uint32_t* count, uint32_t cap, float* samp, int64_t n_px,
@@ -307,7 +306,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_glass(DevScene S, RSettings st, c
         Ray rr;
         rr.o = f3(a.x, a.y, a.z);
         rr.d = f3(b.x, b.y, b.z);
-        Rng rng(1u);
+        Rng rng(__float_as_uint(r4.w));
         const f3 gv = whitted(S, st, rr, f.depth - 1, rng, F, 1, stk, c);
         const f3 T(h.x, h.y, h.z), E(m.x, m.y, m.z);
         f3 v(r4.x, r4.y, r4.z);
@@ -818,16 +817,22 @@ int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, in
     return PCL_OK;
 } PCL_CATCH(ctx)
 
+// Generator draws of one shaded Whitted node (whitted_local): a unit_disc
+// (2 draws) per sample of every point light with a radius.
+static int whitted_draws_per_node(const SceneSlot& sl, const RSettings& st) {
+    int d = 0;
+    for (const LightRec& L : sl.lights)
+        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) d += 2 * st.soft_samples;
+    return d;
+}
+
 // Deferred glass vertices (render_pixel_defer): path tracing, a scene with a
-// dielectric and no point light with a radius (Whitted then draws no random
-// numbers), and buffers that fit: the per-sample values of the frame (12 B
+// dielectric, and buffers that fit: the per-sample values of the frame (12 B
 // per pixel and sample) and a task array sized for an eighth of the samples
 // (128 B each; tasks beyond it are evaluated in place), together under half
 // of the device's free memory.  Returns false (render in place) otherwise.
 static bool defer_buffers(pcl_ctx* ctx, const SceneSlot& sl, const RSettings& st) {
     if (!st.pathtrace || !sl.has_glass || ctx->defer_glass == 0) return false;
-    for (const LightRec& L : sl.lights)
-        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) return false;
     const int64_t n_px = (int64_t)st.width * st.height, nn = (int64_t)st.samples * st.samples;
     if (n_px * nn >= ((int64_t)1 << 32)) return false;   // the task counter is 32-bit
     const size_t samp = (size_t)(n_px * nn * 3);
@@ -884,6 +889,7 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
         db.sink.count = ctx->d_gcount;
         db.sink.cap = (uint32_t)std::min<size_t>(ctx->gtask_cap, ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks
                                                                                        : ctx->gtask_cap);
+        db.sink.draws = whitted_draws_per_node(*sl, st);
         db.samp = ctx->d_samp;
         db.prefix = ctx->d_prefix;
         db.first = ctx->d_first;

@@ -91,6 +91,7 @@ struct Rng {
     PCL_D uint32_t u32() { s ^= s << 13u; s ^= s >> 17u; s ^= s << 5u; return s; }
     PCL_D float unit() { return (float)u32() / 4294967296.0f; }
     PCL_D float sym() { return unit() * 2.0f - 1.0f; }
+    PCL_D void skip(uint32_t n) { for (uint32_t k = 0; k < n; ++k) u32(); }   // n draws, values unused
     // Shirley-Chiu concentric mapping (rng.hpp:28-44)
     PCL_D void unit_disc(float& dx, float& dy) {
         float r1 = unit(), r2 = unit();

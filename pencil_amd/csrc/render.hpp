@@ -220,21 +220,30 @@ PCL_D Ray push_glass(FR* F, int& nf, const Ray& r, const HitRec& rec, const f3& 
 
 // Deferred glass vertices.  A path that hits glass ends there: its sample
 // is R + (T * g - T * emission), g the value of trace_dielectric's Whitted
-// subtree (renderer.hpp:941-949).  When no point light has a radius, Whitted
-// draws no random numbers, so g depends on nothing the path's later samples
-// use: the per-pixel program can record the glass vertex as a task and go on
-// with its next sample, and a separate launch (core.hip: k_glass) evaluates
-// the subtrees of all tasks with full waves instead of one lane while its
+// subtree (renderer.hpp:941-949).  g depends on nothing the path's later
+// samples use except through the generator: Whitted's only draws are the
+// light jitter, two per unit_disc, soft_samples unit_discs per point light
+// with a radius, at every node whose closest hit succeeds -- `draws` per
+// shaded node, the same for every node of the scene.  The subtree's rays do
+// not depend on those draws (reflection and refraction are deterministic),
+// so the per-pixel program can record the glass vertex as a task with the
+// generator state it starts from, walk the subtree's closest hits alone to
+// count its shaded nodes (none of that when draws == 0), advance the
+// generator past the subtree's draws and go on with its next sample; a
+// separate launch (core.hip: k_glass) evaluates the subtrees of all tasks,
+// from the recorded states, with full waves instead of one lane while its
 // wave waits.  A task is 8 float4: the reflection ray, the glass frame
-// (push_glass), T, R, emission, and the sample it completes.
+// (push_glass), T, R, emission, the generator state and the sample it
+// completes.
 struct GlassSink {
     float4* tasks;       // 8 float4 per task
     uint32_t* count;     // tasks appended; may pass cap (those were not deferred)
     uint32_t cap;
+    int32_t draws;       // generator draws per shaded Whitted node
     int32_t pixel, k;    // the sample being traced
     // Wave-aggregated append (every lane of the branch calls it); false when
     // the task array is full and the caller evaluates the vertex itself.
-    PCL_D bool append(const Frame& f, const Ray& rr, const f3& T, const f3& R, const f3& e) {
+    PCL_D bool append(const Frame& f, const Ray& rr, const f3& T, const f3& R, const f3& e, uint32_t rng) {
         const unsigned long long m = __ballot(1);
         const int lane = threadIdx.x & 63;
         const int leader = __ffsll((long long)m) - 1;
@@ -250,7 +259,7 @@ struct GlassSink {
         q[3] = make_float4(f.ro.x, f.ro.y, f.ro.z, f.w1);
         q[4] = make_float4(f.rd.x, f.rd.y, f.rd.z, __int_as_float(pixel));
         q[5] = make_float4(T.x, T.y, T.z, __int_as_float(k));
-        q[6] = make_float4(R.x, R.y, R.z, 0.0f);
+        q[6] = make_float4(R.x, R.y, R.z, __uint_as_float(rng));
         q[7] = make_float4(e.x, e.y, e.z, 0.0f);
         return true;
     }
@@ -407,16 +416,62 @@ struct PathState {
     bool deferred = false;   // the walk ended at a deferred glass vertex
 };
 
+struct NoSink {
+    int32_t draws = 0;
+    PCL_D bool append(const Frame&, const Ray&, const f3&, const f3&, const f3&, uint32_t) { return false; }
+};
+
+// The number of nodes of the Whitted recursion whitted(S, st, r, depth, ., F,
+// nf) evaluates (renderer.hpp:403-822, trace_dielectric :1582-1627) whose
+// closest hit succeeds at depth > 0 -- the nodes that run the light loop and
+// draw -- found by tracing the same rays in the same order without lights,
+// shading or colours.  Uses and changes the frames from F[nf] down.
+PCL_D int whitted_shaded_nodes(const DevScene& S, const RSettings& st, Ray r, int depth, Frame* F, int nf,
+                               Stack& stk) {
+    LaneCount<false> none;   // the rays are counted where the subtree is evaluated
+    int n = 0;
+    while (true) {
+        bool child = false;
+        if (depth > 0) {
+            HitRec rec;
+            if (scene_hit(S, r, 1e-4f, 1e9f, rec, stk, none)) {
+                ++n;
+                int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+                load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+                if (kind == PCL_MAT_METAL && refl > 0.0f) {
+                    Frame& f = F[nf++];
+                    f.type = FR_METAL;
+                    f.depth = depth;
+                    Ray rr;
+                    rr.o = rec.p + rec.normal * st.shadow_bias;
+                    rr.d = r.d.normalized().reflect(rec.normal).normalized();
+                    r = rr;
+                    depth -= 1;
+                    child = true;
+                } else if (kind == PCL_MAT_DIELECTRIC) {
+                    r = push_glass(F, nf, r, rec, albedo, ior, depth, st.shadow_bias);
+                    depth -= 1;
+                    child = true;
+                }
+            }
+        }
+        if (child) continue;
+        // the return path: a reflected glass child is followed by its refraction sibling
+        while (nf > 0 && F[nf - 1].type != FR_GLASS_REFL) --nf;
+        if (nf == 0) return n;
+        Frame& f = F[nf - 1];
+        f.type = FR_GLASS_REFR;
+        r.o = f.ro; r.d = f.rd;
+        depth = f.depth - 1;
+    }
+}
+
 // One iteration of trace_path's bounce loop (its caller checks depth > 0):
 // closest hit, emission, NEE per light, ambient, Russian roulette, the next
 // ray.  Returns false when the walk ends (miss, roulette, glass: the glass
 // subtree runs here as trace_dielectric does).  With st.pathtrace == 2,
 // trace_path_mis (renderer.hpp:1205-1335), which differs only in the light
 // loop (its tangent frame is the diffuse branch's, computed earlier).
-struct NoSink {
-    PCL_D bool append(const Frame&, const Ray&, const f3&, const f3&, const f3&) { return false; }
-};
-
 // `sink`: where a glass vertex may be deferred (GlassSink); P.deferred is
 // then set and P.R is not the sample's value.
 template <class LC, class SINK = NoSink>
@@ -469,7 +524,8 @@ PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& 
     } else if (kind == PCL_MAT_DIELECTRIC) {
         int nf = 0;
         Ray rr = push_glass(F, nf, P.cur, rec, albedo, ior, P.depth, st.shadow_bias);
-        if (sink && sink->append(F[0], rr, P.T, P.R, emission)) {
+        if (sink && sink->append(F[0], rr, P.T, P.R, emission, rng.s)) {
+            if (sink->draws) rng.skip((uint32_t)(sink->draws * whitted_shaded_nodes(S, st, rr, P.depth - 1, F, nf, stk)));
             P.deferred = true;
             return false;
         }

@@ -1,16 +1,17 @@
 """Deferred glass vertices in the path-traced megakernel.
 
-In a scene with a dielectric and no point light with a radius, a path that
-hits glass records the vertex as a task and goes on with its next sample; a
-second launch evaluates the Whitted subtrees of all tasks and a third adds
-each deferred pixel's samples in sample order.  The image and every count
-must be the ones of the in-place program bit for bit: the same samples, the
-same additions in the same order.  The task array can fill up, in which case
-the remaining glass vertices are evaluated in place, also bit for bit.
-
-The scenes' point lights get radius 0 here (the parser's default is 0.5,
-which makes Whitted draw random numbers and rules deferral out); the
-deferred_glass count shows that the deferred program ran.
+A path that hits glass records the vertex as a task with its generator
+state, advances the generator past the draws of the Whitted subtree (soft
+lights: counted by walking the subtree's closest hits) and goes on with its
+next sample; a second launch evaluates the subtrees of all tasks and a third
+adds each deferred pixel's samples in sample order.  The image and every
+count must be the ones of the in-place program bit for bit: the same
+samples, the same draws, the same additions in the same order.  The task
+array can fill up, in which case the remaining glass vertices are evaluated
+in place, also bit for bit.  Scenes run as parsed (point lights at the
+default radius 0.5: soft, 4 jittered samples each) and with every point
+light's radius set to 0 (Whitted then draws nothing); the deferred_glass
+count shows that the deferred program ran.
 """
 import os
 
@@ -55,9 +56,11 @@ def _hard(hs):
     return hs
 
 
-def _scene(name, w, h, side, pathtrace, bounces):
+def _scene(name, w, h, side, pathtrace, bounces, hard):
     path = CORNELL if name == "cornell_c5" else golden(name + ".pecl")
-    hs = _hard(pa.HostScene.from_pecl(path, width=w, height=h, samples=side))
+    hs = pa.HostScene.from_pecl(path, width=w, height=h, samples=side)
+    if hard:
+        _hard(hs)
     st = hs.settings()
     st.pathtrace, st.pathtrace_bounces = pathtrace, bounces
     return hs, st
@@ -81,10 +84,11 @@ CASES = [("cornell_c5", 96, 72, 4, 1, 8), ("cornell_c5", 96, 72, 4, 2, 8), ("cor
          ("three_spheres", 64, 48, 3, 1, 8), ("glass", 64, 48, 2, 1, 6), ("glass", 64, 48, 2, 2, 5)]
 
 
+@pytest.mark.parametrize("hard", [True, False], ids=["hard", "soft"])
 @pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}x{c[2]}-{c[3]}x{c[3]}spp-pt{c[4]}-b{c[5]}" for c in CASES])
-def test_deferred_glass_is_bit_identical(cores, case):
+def test_deferred_glass_is_bit_identical(cores, case, hard):
     c_off, c_on, c_small = cores
-    hs, st = _scene(*case)
+    hs, st = _scene(*case, hard)
     ref, sr = _render(c_off, hs, st)
     assert sr.deferred_glass == 0
     img, s = _render(c_on, hs, st)
@@ -95,35 +99,38 @@ def test_deferred_glass_is_bit_identical(cores, case):
     assert s.deferred_glass == 37
 
 
-def test_deferred_glass_on_the_1m_triangle_cornell_frame(cores):
+@pytest.mark.parametrize("hard", [True, False], ids=["hard", "soft"])
+def test_deferred_glass_on_the_1m_triangle_cornell_frame(cores, hard):
     """The path-traced Cornell + mesh workload's scene at 320x180, 4 spp,
-    8 bounces with Russian roulette: both grids' sizes of image reach the
-    deferred program (a persistent grid needs a large image)."""
+    8 bounces with Russian roulette."""
     from pencil_amd import workloads as wl
     c_off, c_on, _ = cores
     hs, st = wl.cornell_pt(pa, width=320, height=180, samples=2)
-    _hard(hs)
+    if hard:
+        _hard(hs)
     ref, sr = _render(c_off, hs, st)
     img, s = _render(c_on, hs, st)
     _same(img, s, ref, sr)
     assert s.deferred_glass > 0
 
 
-def test_soft_lights_render_in_place(cores):
-    """A point light with a radius makes Whitted draw random numbers (the
-    light jitter), so such scenes are never deferred; the images agree."""
-    c_off, c_on, _ = cores
-    hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=48, height=36, samples=2)
+@pytest.mark.parametrize("soft_samples", [1, 3])
+def test_mixed_lights_and_soft_sample_counts(cores, soft_samples):
+    """One hard and one soft point light, 1 or 3 jittered samples per soft
+    light: the generator skip counts only the soft light's draws."""
+    c_off, c_on, c_small = cores
+    hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=48, height=36, samples=3)
     st = hs.settings()
-    st.pathtrace, st.pathtrace_bounces = 1, 5
+    st.pathtrace, st.pathtrace_bounces, st.soft_shadow_samples = 1, 6, soft_samples
     d = hs.desc()
     assert d.n_lights >= 2
     d.lights[0].radius = 0.0
-    d.lights[1].radius = 0.3   # one soft light is enough
+    d.lights[1].radius = 0.3
     ref, sr = _render(c_off, hs, st)
-    img, s = _render(c_on, hs, st)
-    _same(img, s, ref, sr)
-    assert s.deferred_glass == 0
+    for core in (c_on, c_small):
+        img, s = _render(core, hs, st)
+        _same(img, s, ref, sr)
+        assert s.deferred_glass > 0
 
 
 def test_deferred_glass_persistent_grid_and_many_tasks():
@@ -135,7 +142,6 @@ def test_deferred_glass_persistent_grid_and_many_tasks():
     c_on = _core(PCL_DEFER_GLASS=1, PCL_RENDER_KERNEL=1)
     try:
         hs, st = wl.cornell_pt(pa, width=640, height=360, samples=4)
-        _hard(hs)
         ref, sr = _render(c_off, hs, st)
         img, s = _render(c_on, hs, st)
     finally:
