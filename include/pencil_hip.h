@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PCL_ABI_VERSION 7
+#define PCL_ABI_VERSION 8
 
 enum {
     PCL_OK              = 0,
@@ -278,6 +278,8 @@ typedef struct pcl_batch_stats {
     double        total_ms;
     pcl_wf_config used;
     uint64_t      finished_pixels;  /* pixels whose last samples ran in the drain's finish launch */
+    uint64_t      deferred_glass;   /* (ABI 8) glass vertices of path-traced items whose Whitted subtree
+                                       the finish launch left to its compacted second launch */
 } pcl_batch_stats;
 
 int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const pcl_wf_config* cfg,

@@ -85,7 +85,8 @@ class WfConfig(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("rays_traced", C.c_uint64), ("closest_queries", C.c_uint64), ("anyhit_queries", C.c_uint64),
                 ("iterations", C.c_uint64), ("kernel_ms", C.c_double), ("total_ms", C.c_double),
-                ("used", WfConfig), ("finished_pixels", C.c_uint64)]
+                ("used", WfConfig), ("finished_pixels", C.c_uint64),
+                ("deferred_glass", C.c_uint64)]
 
 
 class SceneInfo(C.Structure):

@@ -273,10 +273,7 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScen
 }
 
 // The deferred glass vertices: one lane per task (grid-stride over 64-task
-// chunks), trace_dielectric's Whitted subtree from the recorded frame and
-// generator state -- whitted() with the frame pushed, as path_step runs it,
-// so the same draws -- then the sample's value R + (T * g - T * emission)
-// into its slot.
+// chunks), glass_task's subtree and sample value into the sample's slot.
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock, 3) void k_glass(DevScene S, RSettings st, const float4* __restrict__ tasks,
                                                      const uint32_t* count, uint32_t cap, float* samp, int64_t n_px,
@@ -294,24 +291,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_glass(DevScene S, RSettings st, c
         const uint32_t t = base + (threadIdx.x & 63);
         if (t >= n) continue;
         const float4* q = tasks + (size_t)t * 8;
-        const float4 a = q[0], b = q[1], d = q[2], e = q[3], g4 = q[4], h = q[5], r4 = q[6], m = q[7];
-        Frame& f = F[0];
-        f.depth = __float_as_int(a.w);
-        f.type = __float_as_int(b.w);
-        f.albedo = f3(d.x, d.y, d.z);
-        f.w0 = d.w;
-        f.ro = f3(e.x, e.y, e.z);
-        f.w1 = e.w;
-        f.rd = f3(g4.x, g4.y, g4.z);
-        Ray rr;
-        rr.o = f3(a.x, a.y, a.z);
-        rr.d = f3(b.x, b.y, b.z);
-        Rng rng(__float_as_uint(r4.w));
-        const f3 gv = whitted(S, st, rr, f.depth - 1, rng, F, 1, stk, c);
-        const f3 T(h.x, h.y, h.z), E(m.x, m.y, m.z);
-        f3 v(r4.x, r4.y, r4.z);
-        v += T * gv - T * E;   // P.R += P.T * g - P.T * emission
-        const int64_t p = __float_as_int(g4.w), k = __float_as_int(h.w);
+        const f3 v = glass_task(S, st, q, F, stk, c);
+        const int64_t p = __float_as_int(q[4].w), k = __float_as_int(q[5].w);
         float* o = samp + (k * n_px + p) * 3;
         o[0] = v.x; o[1] = v.y; o[2] = v.z;
     }
@@ -514,6 +495,8 @@ int pcl_init(int device, pcl_ctx** out) try {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->cus = prop.multiProcessorCount;
     ctx->wf = new pcl::wf::Tracer();
+    ctx->wf->defer_glass = ctx->defer_glass;
+    ctx->wf->defer_tasks = ctx->defer_tasks;
     *out = ctx.release();
     return PCL_OK;
 } PCL_CATCH(nullptr)
@@ -818,7 +801,8 @@ int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, in
 } PCL_CATCH(ctx)
 
 // Generator draws of one shaded Whitted node (whitted_local): a unit_disc
-// (2 draws) per sample of every point light with a radius.
+// (2 draws) per sample of every point light with a radius (the host copy of
+// render.hpp's whitted_draws).
 static int whitted_draws_per_node(const SceneSlot& sl, const RSettings& st) {
     int d = 0;
     for (const LightRec& L : sl.lights)
@@ -1010,6 +994,7 @@ static int batch_prepare(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, s
         r.st = to_dev(&b.settings);
         r.out = b.d_out_rgb;
         r.scene = i;
+        r.glass = sl->has_glass ? 1 : 0;
         r.pix_begin = begin;
         begin += pcl::wf::padded_pixels(b.settings.width, b.settings.height);
         its.push_back(r);
@@ -1060,6 +1045,7 @@ int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const
         stats->total_ms = now_ms() - t0;
         stats->used = to_abi(rs.cfg);
         stats->finished_pixels = rs.finished;
+        stats->deferred_glass = rs.deferred;
     }
     return PCL_OK;
 } PCL_CATCH(ctx)

@@ -241,6 +241,7 @@ struct GlassSink {
     uint32_t cap;
     int32_t draws;       // generator draws per shaded Whitted node
     int32_t pixel, k;    // the sample being traced
+    int32_t item = 0;    // batch item of the pixel (the batched tracer's finish launch)
     // Wave-aggregated append (every lane of the branch calls it); false when
     // the task array is full and the caller evaluates the vertex itself.
     PCL_D bool append(const Frame& f, const Ray& rr, const f3& T, const f3& R, const f3& e, uint32_t rng) {
@@ -260,7 +261,7 @@ struct GlassSink {
         q[4] = make_float4(f.rd.x, f.rd.y, f.rd.z, __int_as_float(pixel));
         q[5] = make_float4(T.x, T.y, T.z, __int_as_float(k));
         q[6] = make_float4(R.x, R.y, R.z, __uint_as_float(rng));
-        q[7] = make_float4(e.x, e.y, e.z, 0.0f);
+        q[7] = make_float4(e.x, e.y, e.z, __int_as_float(item));
         return true;
     }
 };
@@ -586,22 +587,23 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
     return acc / (float)(n * n);
 }
 
-// Glass subtrees evaluated by a later launch (trace_path with a GlassSink).
-// render_pixel's loop, except that once a sample is deferred the sum stops:
-// acc then holds the sum of the samples before the first deferred one
-// (`first`), and every later sample evaluated here is stored in samp (sample
-// k of this pixel at samp[k * stride]), the deferred ones being filled in by
-// the glass launch; a last launch adds them to acc in sample order, so the
-// pixel gets render_pixel's additions in render_pixel's order.  Returns
-// true, with the pixel value in px, when no sample was deferred.
+// Glass subtrees evaluated by a later launch (trace_path with a GlassSink),
+// from sample k0 on: `rng` is the pixel's stream as sample k0 finds it and
+// `acc` the sum of samples 0..k0-1.  render_pixel's loop, except that once a
+// sample is deferred the sum stops: acc then holds the sum of the samples
+// before the first deferred one (`first`), and every later sample evaluated
+// here is stored in samp (sample k of this pixel at samp[k * stride]), the
+// deferred ones being filled in by the glass launch; a last launch adds them
+// to acc in sample order, so the pixel gets render_pixel's additions in
+// render_pixel's order.  Returns true, with the pixel value in px, when no
+// sample was deferred (first = n^2).
 template <class LC>
-PCL_D bool render_pixel_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F,
-                              Stack& stk, LC& c, GlassSink& sink, float* samp, size_t stride, f3& px, int& first) {
+PCL_D bool render_samples_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, int k0,
+                                Rng& rng, f3 acc, Frame* F, Stack& stk, LC& c, GlassSink& sink, float* samp,
+                                size_t stride, f3& px, int& first) {
     const int W = st.width, H = st.height, n = st.samples;
-    Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
-    f3 acc(0, 0, 0);
     first = n * n;
-    for (int k = 0; k < n * n; ++k) {
+    for (int k = k0; k < n * n; ++k) {
         const int sy = k / n, sx = k - sy * n;
         float u = (i + (sx + 0.5f) / n) / (float)W;
         float v = 1.0f - (j + (sy + 0.5f) / n) / (float)H;
@@ -626,6 +628,52 @@ PCL_D bool render_pixel_defer(const DevScene& S, const CamRec& C, const RSetting
     }
     px = (first == n * n) ? acc / (float)(n * n) : acc;
     return first == n * n;
+}
+
+// The whole pixel (render_pixel's generator seed, samples 0..n^2-1).
+template <class LC>
+PCL_D bool render_pixel_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F,
+                              Stack& stk, LC& c, GlassSink& sink, float* samp, size_t stride, f3& px, int& first) {
+    Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
+    return render_samples_defer(S, C, st, i, j, 0, rng, f3(0, 0, 0), F, stk, c, sink, samp, stride, px, first);
+}
+
+// Generator draws of one shaded Whitted node (whitted_local): a unit_disc
+// (2 draws) per sample of every point light with a radius.
+PCL_D int whitted_draws(const DevScene& S, const RSettings& st) {
+    int d = 0;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const LightRec L = load_light(S.lights, li);
+        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) d += 2 * st.soft_samples;
+    }
+    return d;
+}
+
+// One deferred glass vertex (a GlassSink task, 8 float4 at q): the Whitted
+// subtree from the recorded frame and generator state -- whitted() with the
+// frame pushed, as path_step runs it, so the same draws -- and the sample's
+// value R + (T * g - T * emission).  The task's pixel, sample and item are
+// the ints in q[4].w, q[5].w and q[7].w.
+template <class LC>
+PCL_D f3 glass_task(const DevScene& S, const RSettings& st, const float4* q, Frame* F, Stack& stk, LC& c) {
+    const float4 a = q[0], b = q[1], d = q[2], e = q[3], g4 = q[4], h = q[5], r4 = q[6], m = q[7];
+    Frame& f = F[0];
+    f.depth = __float_as_int(a.w);
+    f.type = __float_as_int(b.w);
+    f.albedo = f3(d.x, d.y, d.z);
+    f.w0 = d.w;
+    f.ro = f3(e.x, e.y, e.z);
+    f.w1 = e.w;
+    f.rd = f3(g4.x, g4.y, g4.z);
+    Ray rr;
+    rr.o = f3(a.x, a.y, a.z);
+    rr.d = f3(b.x, b.y, b.z);
+    Rng rng(__float_as_uint(r4.w));
+    const f3 gv = whitted(S, st, rr, f.depth - 1, rng, F, 1, stk, c);
+    const f3 T(h.x, h.y, h.z), E(m.x, m.y, m.z);
+    f3 v(r4.x, r4.y, r4.z);
+    v += T * gv - T * E;   // P.R += P.T * g - P.T * emission
+    return v;
 }
 
 // render_pixel from sample k0 on: `rng` is the pixel's stream as sample k0

@@ -816,14 +816,29 @@ __global__ __launch_bounds__(kWfBlock) void k_finish_list(Ctx C, int n_slots) {
     }
 }
 
+// Deferred glass vertices of the finish launch: the task array (GlassSink),
+// sample k of listed pixel p (entry from + p) at samp[(k * n_px + p) * 3],
+// and per listed pixel the sum of its samples before the first deferred one
+// and that sample's index.
+struct FinishDefer {
+    GlassSink sink;
+    float* samp;
+    float* prefix;
+    int32_t* first;
+    int64_t n_px;
+};
+
 // Finishes the listed pixels [from .. to) one per lane with the per-pixel
 // program (render_samples, the megakernel's loop from the record's next
 // sample on).  The tail of a job -- few pixels with long sample chains, such
-// as pixels that see glass -- then runs without a barrier per query.
+// as pixels that see glass -- then runs without a barrier per query.  With
+// DEFER (path-traced jobs) a path that hits glass leaves the vertex's Whitted
+// subtree to k_finish_glass, as the megakernel's deferred program does, and a
+// pixel that deferred a sample is left to k_finish_sum.
 // Register budget as the megakernel's for the same mode.
-template <int PT>
+template <int PT, bool DEFER>
 __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, const int4* meta, const float4* accs,
-                                                                    int from, int to) {
+                                                                    int from, int to, FinishDefer fd) {
     __shared__ unsigned long long lds[kStackK * kWfBlock];
     const int e = from + blockIdx.x * blockDim.x + threadIdx.x;
     LaneCount<false> c;
@@ -838,14 +853,87 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
         stk.set_stride(kWfBlock);
         const int W = it.st.width;
         Rng rng((uint32_t)m.w);
-        const f3 px = render_samples<PT>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F, stk, c);
-        float* o = it.out + (size_t)m.y * 3;
-        o[0] = px.x; o[1] = px.y; o[2] = px.z;
+        f3 px;
+        bool done = true;
+        if constexpr (DEFER) {
+            const int64_t p = e - from;
+            GlassSink sink = fd.sink;
+            sink.draws = whitted_draws(D, it.st);
+            sink.pixel = (int32_t)p;
+            sink.item = m.x;
+            int first;
+            done = render_samples_defer(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F, stk, c, sink,
+                                        fd.samp + p * 3, (size_t)fd.n_px * 3, px, first);
+            fd.first[p] = first;
+            if (!done) {
+                float* q = fd.prefix + p * 3;
+                q[0] = px.x; q[1] = px.y; q[2] = px.z;
+            }
+        } else {
+            px = render_samples<PT>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F, stk, c);
+        }
+        if (done) {
+            float* o = it.out + (size_t)m.y * 3;
+            o[0] = px.x; o[1] = px.y; o[2] = px.z;
+        }
     }
     unsigned long long* line = &C.q.stats[(1 + (blockIdx.x & (kStripes - 1))) * kStatLine];
     wave_add_u64(c.rays, line + ST_RAYS);
     wave_add_u64(c.closest, line + ST_CLOSEST);
     wave_add_u64(c.anyhit, line + ST_ANY);
+}
+
+// The finish launch's deferred glass vertices: one lane per task (grid-stride
+// over 64-task chunks), glass_task's subtree in the task's item's scene, the
+// sample's value into its slot.
+__global__ __launch_bounds__(kWfBlock, 3) void k_finish_glass(Ctx C, const float4* __restrict__ tasks,
+                                                             const uint32_t* count, uint32_t cap, float* samp,
+                                                             int64_t n_px) {
+    __shared__ unsigned long long lds[kStackK * kWfBlock];
+    const uint32_t n = min(*count, cap);
+    LaneCount<false> c;
+    alignas(16) unsigned char fbuf[sizeof(Frame) * kMaxFrames];
+    Frame* F = reinterpret_cast<Frame*>(fbuf);
+    Stack stk;
+    stk.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
+    stk.set_stride(kWfBlock);
+    const uint32_t wave = (blockIdx.x * kWfBlock + threadIdx.x) >> 6, waves = (gridDim.x * kWfBlock) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
+        const uint32_t t = base + (threadIdx.x & 63);
+        if (t >= n) continue;
+        const float4* q = tasks + (size_t)t * 8;
+        const ItemRec& it = C.items[__float_as_int(q[7].w)];
+        const f3 v = glass_task(C.scenes[it.scene], it.st, q, F, stk, c);
+        const int64_t p = __float_as_int(q[4].w), k = __float_as_int(q[5].w);
+        float* o = samp + (k * n_px + p) * 3;
+        o[0] = v.x; o[1] = v.y; o[2] = v.z;
+    }
+    unsigned long long* line = &C.q.stats[(1 + (blockIdx.x & (kStripes - 1))) * kStatLine];
+    wave_add_u64(c.rays, line + ST_RAYS);
+    wave_add_u64(c.closest, line + ST_CLOSEST);
+    wave_add_u64(c.anyhit, line + ST_ANY);
+}
+
+// Listed pixels that deferred a sample: the prefix plus samples first..n^2-1
+// in order, divided by n^2 -- render_samples' sum, same additions, same order.
+__global__ __launch_bounds__(256) void k_finish_sum(Ctx C, const int4* __restrict__ meta, int from,
+                                                   const float* __restrict__ samp, const float* __restrict__ prefix,
+                                                   const int32_t* __restrict__ first, int64_t n_px) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_px) return;
+    const int k0 = first[p];
+    const int4 m = meta[from + p];
+    const ItemRec& it = C.items[m.x];
+    const int nn = it.st.samples * it.st.samples;
+    if (k0 >= nn) return;
+    f3 acc(prefix[p * 3], prefix[p * 3 + 1], prefix[p * 3 + 2]);
+    for (int k = k0; k < nn; ++k) {
+        const float* s = samp + ((int64_t)k * n_px + p) * 3;
+        acc += f3(s[0], s[1], s[2]);
+    }
+    const f3 px = acc / (float)nn;
+    float* o = it.out + (size_t)m.y * 3;
+    o[0] = px.x; o[1] = px.y; o[2] = px.z;
 }
 
 // Clears the job counters (a kernel, not a memset, so it may sit in a graph).
@@ -891,6 +979,48 @@ void Engine::release() {
     bufs.clear();
     cap_slots = 0;
     cap_items = 0;
+    for (void* p : {(void*)d_gtasks, (void*)d_gcount, (void*)d_samp, (void*)d_prefix, (void*)d_first})
+        if (p) (void)hipFree(p);
+    d_gtasks = nullptr; d_gcount = nullptr; d_samp = nullptr; d_prefix = nullptr; d_first = nullptr;
+    gtask_cap = samp_cap = px_cap = 0;
+}
+
+// The finish launch's deferred-glass buffers: 12 B per listed pixel and
+// sample, a task array sized for half of those samples (the listed pixels are
+// the job's longest, often glass-heavy; tasks beyond it are evaluated in
+// place), 16 B per listed pixel; together under half of the device's free
+// memory.  The task counter is 32-bit, so n_px * nn stays below 2^32.
+bool Engine::defer_buffers(int64_t n_px, int64_t nn) {
+    if (n_px <= 0 || n_px * nn >= ((int64_t)1 << 32)) return false;
+    const size_t samp = (size_t)(n_px * nn * 3);
+    const size_t tasks = defer_tasks > 0 ? (size_t)defer_tasks
+                                         : std::min<size_t>((size_t)(n_px * nn / 2 + 4096), (size_t)1 << 27);
+    size_t need = 0;
+    if (samp > samp_cap) need += samp * sizeof(float);
+    if (tasks > gtask_cap) need += tasks * 8 * sizeof(float4);
+    if ((size_t)n_px > px_cap) need += (size_t)n_px * 16;
+    if (need) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || need > free_b / 2) return false;
+    }
+    if (!d_gcount && hipMalloc((void**)&d_gcount, 256) != hipSuccess) { d_gcount = nullptr; return false; }
+    auto grow = [](auto** p, size_t& cap, size_t want, size_t elem) -> bool {
+        if (want <= cap) return true;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        cap = 0;
+        if (hipMalloc((void**)p, want * elem) != hipSuccess) { *p = nullptr; return false; }
+        cap = want;
+        return true;
+    };
+    size_t pxc = px_cap, pxc2 = px_cap;
+    if (!grow(&d_samp, samp_cap, samp, sizeof(float)) || !grow(&d_gtasks, gtask_cap, tasks, 8 * sizeof(float4)) ||
+        !grow(&d_prefix, pxc, (size_t)n_px, 3 * sizeof(float)) || !grow(&d_first, pxc2, (size_t)n_px, sizeof(int32_t))) {
+        px_cap = std::min(pxc, pxc2);
+        return false;
+    }
+    px_cap = std::min(pxc, pxc2);
+    return true;
 }
 
 int Engine::alloc(size_t bytes, void** p, std::string& err) {
@@ -1015,12 +1145,44 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     bool any_path = false, any_whitted = false;
     for (const auto& it : its) (it.st.pathtrace ? any_path : any_whitted) = true;
     // finish kernel over records [from .. to) on stream st
+    bool any_glass = false;
+    int64_t nn_max = 1;
+    for (const auto& it : its) {
+        any_glass |= it.glass != 0;
+        nn_max = std::max<int64_t>(nn_max, (int64_t)it.st.samples * it.st.samples);
+    }
+    // Deferred glass vertices in the finish launch (as the megakernel's): for
+    // path-traced jobs with a dielectric, when the buffers fit.
+    bool deferred = false;
     auto finish = [&](const int4* m, const float4* a, uint32_t from, uint32_t to, hipStream_t st) -> hipError_t {
         if (to <= from) return hipSuccess;
         const dim3 grid((to - from + kWfBlock - 1) / kWfBlock);
-        if (any_path && !any_whitted) hipLaunchKernelGGL(k_finish<1>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
-        else if (!any_path) hipLaunchKernelGGL(k_finish<0>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
-        else hipLaunchKernelGGL(k_finish<-1>, grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to);
+        FinishDefer fd{};
+        const int64_t n_px = (int64_t)(to - from);
+        deferred = any_path && !any_whitted && any_glass && defer_glass != 0 && defer_buffers(n_px, nn_max);
+        if (deferred) {
+            hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, d_gcount, 0u);
+            fd.sink.tasks = d_gtasks;
+            fd.sink.count = d_gcount;
+            fd.sink.cap = (uint32_t)std::min<size_t>(gtask_cap, defer_tasks > 0 ? (size_t)defer_tasks : gtask_cap);
+            fd.samp = d_samp;
+            fd.prefix = d_prefix;
+            fd.first = d_first;
+            fd.n_px = n_px;
+            hipLaunchKernelGGL((k_finish<1, true>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+            int nb = 0;
+            hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_finish_glass, kWfBlock, 0);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_finish_glass, dim3(cus * std::max(nb, 1)), dim3(kWfBlock), 0, st, c, d_gtasks,
+                               d_gcount, fd.sink.cap, d_samp, n_px);
+            hipLaunchKernelGGL(k_finish_sum, dim3((unsigned)((n_px + 255) / 256)), dim3(256), 0, st, c, m, (int)from,
+                               d_samp, d_prefix, d_first, n_px);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+            return hipMemcpyAsync(&h_pending[1], d_gcount, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        }
+        if (any_path && !any_whitted) hipLaunchKernelGGL((k_finish<1, false>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+        else if (!any_path) hipLaunchKernelGGL((k_finish<0, false>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+        else hipLaunchKernelGGL((k_finish<-1, false>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
         return hipGetLastError();
     };
     // timing events, destroyed on every return path
@@ -1105,6 +1267,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     out.iterations = h_st[ST_ITERS];
     out.graph_launches = launches;
     out.finished = n_finish;
+    out.deferred = deferred ? std::min<uint64_t>(h_pending[1], std::min<size_t>(gtask_cap, defer_tasks > 0 ? (size_t)defer_tasks : gtask_cap)) : 0;
     out.ms = ms;
     out.cfg = cfg;
     return PCL_OK;
@@ -1130,6 +1293,10 @@ size_t Tracer::cap_slots() const {
 
 int Tracer::run(int device, hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
                 const Config& cfg_in, int cus, RunStats& out, std::string& err) {
+    for (Engine& e : eng) {
+        e.defer_glass = defer_glass;
+        e.defer_tasks = defer_tasks;
+    }
     Config cfg = cfg_in;
     int64_t total = 0;
     for (const auto& it : items) total += padded_pixels(it.st.width, it.st.height);
@@ -1218,6 +1385,7 @@ int Tracer::run(int device, hipStream_t stream, const std::vector<ItemRec>& item
         out.anyhit += rs[i].anyhit;
         out.iterations = std::max(out.iterations, rs[i].iterations);
         out.finished += rs[i].finished;
+        out.deferred += rs[i].deferred;
         out.graph_launches += rs[i].graph_launches;
         float m = 0.0f;
         WF_TRY(hipEventElapsedTime(&m, ev_fork, ev_done[i]));

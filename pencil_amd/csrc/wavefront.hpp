@@ -23,7 +23,7 @@ struct ItemRec {
     RSettings st;
     float* out;
     int32_t scene;
-    int32_t pad;
+    int32_t glass;       // the scene has a dielectric material
     int64_t pix_begin;   // first job pixel index of this item
 };
 
@@ -121,6 +121,7 @@ inline int64_t part_pixels(int64_t total, int part, int parts) {
 struct RunStats {
     unsigned long long rays = 0, closest = 0, anyhit = 0, iterations = 0;
     unsigned long long finished = 0;   // pixels completed by the drain's finish kernel
+    unsigned long long deferred = 0;   // glass vertices the finish kernel deferred
     int graph_launches = 0;
     float ms = 0.0f;
     Config cfg;
@@ -140,11 +141,26 @@ struct Engine {
     int graph_part = -1, graph_parts = 0;
     uint32_t* h_pending = nullptr;      // pinned, kStatusWords per graph launch in flight
     hipEvent_t ev_pending[2] = {nullptr, nullptr};
+    // Deferred glass vertices of the finish launch (path-traced jobs): the
+    // task array and its counter, the per-sample values of the listed pixels
+    // and their prefix sums / first deferred samples; grown on demand and
+    // kept until release().
+    float4* d_gtasks = nullptr;
+    uint32_t* d_gcount = nullptr;
+    float* d_samp = nullptr;
+    float* d_prefix = nullptr;
+    int32_t* d_first = nullptr;
+    size_t gtask_cap = 0, samp_cap = 0, px_cap = 0;
+    int defer_glass = 1;   // 0: the finish launch evaluates glass vertices in place
+    int64_t defer_tasks = 0;   // > 0: task array capacity limit (tests)
 
     ~Engine();
     void release();
     int alloc(size_t bytes, void** p, std::string& err);
     int ensure(int slots, int items, std::string& err);
+    // Sizes the deferred-glass buffers for n_px listed pixels of at most nn
+    // samples; false (finish in place) when they would not fit.
+    bool defer_buffers(int64_t n_px, int64_t nn);
     // Runs part `part` of `parts` of the job on `stream`; `done`, when given,
     // is recorded on the stream after the part's last kernel.
     int run(hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
@@ -160,6 +176,8 @@ struct Tracer {
     Engine eng[kMaxParts];
     hipStream_t side[kMaxParts] = {};
     hipEvent_t ev_fork = nullptr, ev_done[kMaxParts] = {};
+    int defer_glass = 1;       // handed to the engines (Engine::defer_glass)
+    int64_t defer_tasks = 0;
 
     ~Tracer();
     void release();

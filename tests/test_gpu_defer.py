@@ -1,4 +1,5 @@
-"""Deferred glass vertices in the path-traced megakernel.
+"""Deferred glass vertices in the path-traced megakernel and in the batched
+tracer's finish launch.
 
 A path that hits glass records the vertex as a task with its generator
 state, advances the generator past the draws of the Whitted subtree (soft
@@ -149,3 +150,87 @@ def test_deferred_glass_persistent_grid_and_many_tasks():
         c_on.close()
     _same(img, s, ref, sr)
     assert s.deferred_glass > 100_000
+
+
+def _batch_case(core, items, cfg):
+    """Renders items [(HostScene, Settings)] as one batched job; returns the
+    images and the job's stats."""
+    sids, bufs, jobs = [], [], []
+    try:
+        for hs, st in items:
+            sid = core.upload(hs.desc())
+            sids.append(sid)
+            d = core.alloc(st.width * st.height * 12)
+            bufs.append(d)
+            jobs.append((sid, hs.camera(), st, d))
+        bs = core.render_batch(jobs, cfg)
+        imgs = []
+        for (_, st), d in zip(items, bufs):
+            out = np.empty((st.height, st.width, 3), dtype=np.float32)
+            core.d2h(out, d)
+            imgs.append(out)
+        return imgs, bs
+    finally:
+        for d in bufs:
+            core.free(d)
+        for sid in sids:
+            core.free_scene(sid)
+
+
+def _batch_items(hard, whitted):
+    items = []
+    for name, w, h, side, pt, b in (("cornell_c5", 96, 72, 3, 1, 8), ("glass", 64, 48, 2, 2, 6),
+                                    ("three_spheres", 40, 30, 4, 1, 8)):
+        items.append(_scene(name, w, h, side, pt, b, hard))
+    if whitted:
+        hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=48, height=36, samples=2)
+        items.append((hs, hs.settings()))
+    return items
+
+
+@pytest.mark.parametrize("drain", [900, 1 << 30])
+@pytest.mark.parametrize("streams", [1, 3])
+@pytest.mark.parametrize("hard", [True, False], ids=["hard", "soft"])
+def test_batched_finish_defers_glass_bit_for_bit(cores, hard, streams, drain):
+    """The batched tracer's drain: its finish launch defers the glass vertices
+    of path-traced jobs (three scenes of different sizes and sample counts,
+    path and MIS) to a second launch as the megakernel does.  Images and
+    counts equal the plain in-place renders bit for bit, with the task array
+    holding every vertex and holding 37; a drain at once (threshold 2^30)
+    leaves nearly every sample to the finish launch."""
+    c_off, c_on, c_small = cores
+    items = _batch_items(hard, whitted=False)
+    refs, rst = [], []
+    for hs, st in items:
+        img, s = _render(c_off, hs, st)
+        refs.append(img)
+        rst.append(s)
+    cfg = pa.WfConfig(slots=6000, blocks_per_cu=2, stack_k=16, iters_per_graph=1, drain_below=drain,
+                      streams=streams)
+    deferred = []
+    for core in (c_off, c_on, c_small):
+        imgs, bs = _batch_case(core, items, cfg)
+        for img, ref in zip(imgs, refs):
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+        assert bs.rays_traced == sum(s.rays_traced for s in rst)
+        assert bs.closest_queries == sum(s.closest_queries for s in rst)
+        assert bs.anyhit_queries == sum(s.anyhit_queries for s in rst)
+        assert bs.finished_pixels > 0
+        deferred.append(bs.deferred_glass)
+    # in place; every vertex; at most 37 per part (each part's finish launch)
+    assert deferred[0] == 0
+    assert deferred[1] > 37 * streams
+    assert 0 < deferred[2] <= 37 * streams
+
+
+def test_batched_mixed_job_finishes_in_place(cores):
+    """A job that mixes Whitted and path-traced items finishes its drain in
+    place (deferral is for path-traced jobs); the images are the plain ones."""
+    c_off, c_on, _ = cores
+    items = _batch_items(False, whitted=True)
+    refs = [_render(c_off, hs, st)[0] for hs, st in items]
+    cfg = pa.WfConfig(slots=6000, blocks_per_cu=2, stack_k=16, iters_per_graph=1, drain_below=1 << 30, streams=1)
+    imgs, bs = _batch_case(c_on, items, cfg)
+    for img, ref in zip(imgs, refs):
+        assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    assert bs.finished_pixels > 0 and bs.deferred_glass == 0

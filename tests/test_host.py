@@ -149,7 +149,7 @@ def test_c_abi_exports_every_declared_symbol():
     missing = declared - exported
     assert not missing, missing
     lib = pa.hip_lib()
-    assert lib.pcl_abi_version() == 7
+    assert lib.pcl_abi_version() == 8
     for name in declared:
         assert hasattr(lib, name)
 
