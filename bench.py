@@ -363,45 +363,63 @@ def cpu_stress(queries, threads, model):
 
 
 # ---------------------------------------------------------------------------
-# glass_defer: the Cornell + mesh frame with a hard point light (radius 0),
-# plain renders with the glass vertices evaluated in place and deferred
-# (not a configured workload: the configured scene's light has the parser's
-# default radius 0.5, whose jitter rules deferral out)
+# glass_defer: the Cornell + mesh frame, glass vertices evaluated in place
+# and deferred, plain and batched, soft and hard light (not run by default)
 # ---------------------------------------------------------------------------
 
 def leg_glass_defer(core, pa, args, threads, model):
-    hs, st = wl.cornell_pt(pa, samples=args.pt_samples)
-    desc, cam = hs.desc(), hs.camera()
-    for i in range(desc.n_lights):
-        desc.lights[i].radius = 0.0
-    W, H = st.width, st.height
-    out, imgs = {}, {}
-    for name, env in (("in_place", "0"), ("deferred", "1")):
-        os.environ["PCL_DEFER_GLASS"] = env
-        c = pa.Core(0)
-        try:
-            sid = c.upload(desc)
-            d = c.alloc(W * H * 12)
-            c.render_device(sid, cam, st, d)   # warm-up (buffers)
-            t0 = time.perf_counter()
-            s = c.render_device(sid, cam, st, d)
-            ms = (time.perf_counter() - t0) * 1000.0
-            img = np.empty((H, W, 3), dtype=np.float32)
-            c.d2h(img, d)
-            imgs[name] = img
-            q = s.closest_queries + s.anyhit_queries
-            out[name] = {"ms": ms, "kernel_ms": s.kernel_ms, "mrays_s": q / ms / 1e3, "queries": q,
-                         "deferred_glass_vertices": s.deferred_glass}
-            log(f"glass_defer {name}: {ms:.1f} ms, {q / ms / 1e3:.1f} Mrays/s, {s.deferred_glass} deferred")
-            c.free(d)
-            c.free_scene(sid)
-        finally:
-            c.close()
-    os.environ.pop("PCL_DEFER_GLASS", None)
-    out["workload"] = (f"Cornell box + {desc.n_prims - 14}-tri mesh with the point light's radius set to 0, {W}x{H}, "
-                       f"{st.samples ** 2} spp, path tracing, 8 bounces, Russian roulette after 3 (plain renders)")
-    out["speedup_deferred"] = out["in_place"]["ms"] / out["deferred"]["ms"]
-    out["bit_identical"] = bool(np.array_equal(imgs["in_place"].view(np.uint32), imgs["deferred"].view(np.uint32)))
+    """Glass vertices evaluated in place vs deferred to a second launch, for
+    the plain megakernel and the batched tracer's finish launch, on the
+    path-traced Cornell + mesh frame as configured (soft point light) and with
+    the light's radius set to 0.  Every image must be the same bit for bit."""
+    cfg_s = args.batched_config or "2073600,6,16,4,0,-1,3"
+    cfg = pa.WfConfig(*[int(x) for x in cfg_s.split(",")])
+    out, imgs = {}, []
+    for light in ("soft", "hard"):
+        hs, st = wl.cornell_pt(pa, samples=args.pt_samples)
+        desc, cam = hs.desc(), hs.camera()
+        if light == "hard":
+            for i in range(desc.n_lights):
+                desc.lights[i].radius = 0.0
+        W, H = st.width, st.height
+        res = {}
+        for name, env in (("in_place", "0"), ("deferred", "1")):
+            os.environ["PCL_DEFER_GLASS"] = env
+            c = pa.Core(0)
+            try:
+                sid = c.upload(desc)
+                d = c.alloc(W * H * 12)
+                r = {}
+                for path in ("plain", "batched"):
+                    run = (lambda: c.render_device(sid, cam, st, d)) if path == "plain" else \
+                        (lambda: c.render_batch([(sid, cam, st, d)], cfg))
+                    run()   # warm-up (buffers, graph)
+                    t0 = time.perf_counter()
+                    s = run()
+                    ms = (time.perf_counter() - t0) * 1000.0
+                    img = np.empty((H, W, 3), dtype=np.float32)
+                    c.d2h(img, d)
+                    imgs.append(img)
+                    q = s.closest_queries + s.anyhit_queries
+                    r[path] = {"ms": ms, "kernel_ms": s.kernel_ms, "mrays_s": q / ms / 1e3, "queries": q,
+                               "deferred_glass_vertices": s.deferred_glass}
+                    log(f"glass_defer {light} {path} {name}: {ms:.1f} ms, {q / ms / 1e3:.1f} Mrays/s, "
+                        f"{s.deferred_glass} deferred")
+                res[name] = r
+                c.free(d)
+                c.free_scene(sid)
+            finally:
+                c.close()
+        os.environ.pop("PCL_DEFER_GLASS", None)
+        res["speedup_deferred_plain"] = res["in_place"]["plain"]["ms"] / res["deferred"]["plain"]["ms"]
+        res["speedup_deferred_batched"] = res["in_place"]["batched"]["ms"] / res["deferred"]["batched"]["ms"]
+        out[light] = res
+    out["workload"] = (f"Cornell box + {desc.n_prims - 14}-tri mesh, {W}x{H}, {st.samples ** 2} spp, path tracing, "
+                       f"8 bounces, Russian roulette after 3; the point light as configured (radius 0.5: soft, "
+                       f"jittered) and with its radius set to 0 (hard); plain megakernel and batched tracer "
+                       f"({cfg_s})")
+    out["bit_identical"] = bool(all(np.array_equal(imgs[k].view(np.uint32), imgs[k - k % 4].view(np.uint32))
+                                    for k in range(len(imgs))))
     return rounded(out)
 
 
@@ -433,6 +451,7 @@ def leg_cornell_pt(core, pa, args, threads, model):
             st_t.samples = 2
             cfg, tune_ms = core.autotune([(sid, cam, st_t, d_b)])
             log(f"cornell_pt autotune at 2x2 spp: {cfg_str(cfg)} -> {tune_ms:.1f} ms")
+        core.render_batch([(sid, cam, st, d_b)], cfg)   # warm-up at full spp (the drain's buffers)
         t0 = time.perf_counter()
         bs = core.render_batch([(sid, cam, st, d_b)], cfg)
         wf_ms = (time.perf_counter() - t0) * 1000.0
@@ -764,7 +783,7 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="skip the batched leg of the headline workload")
     ap.add_argument("--legs", default="cornell_pt,batch64,inverse",
                     help="further workloads to run after the headline one (comma list, or 'none'); "
-                         "glass_defer (not run by default) measures deferred glass vertices on a hard-light frame")
+                         "glass_defer (not run by default) times glass vertices in place vs deferred")
     ap.add_argument("--pt-samples", type=int, default=wl.CORNELL_PT["samples"],
                     help="samples per side for the cornell_pt leg (16 = the workload's 256 spp; less for profiling)")
     ap.add_argument("--batched-config", default="",

@@ -258,7 +258,8 @@ typedef struct pcl_wf_config {
     int32_t iters_per_graph;  /* wavefront iterations captured per hipGraph replay */
     int32_t drain_below;      /* once every job pixel is taken and fewer slots than this are
                                  active, each slot stops at its next sample boundary and one
-                                 per-pixel launch finishes the remaining samples; -1 = never */
+                                 per-pixel launch finishes the remaining samples; -1 = never,
+                                 0 = the traversal grids' lanes (CUs x blocks_per_cu x 128) */
     int32_t inline_shadows;   /* a vertex with at most this many shadow rays traces them in the
                                  closest-hit kernel instead of the shadow queue; -1 = never */
     int32_t streams;          /* engines the job's 64-pixel tiles are dealt to (interleaved), each
@@ -284,8 +285,11 @@ typedef struct pcl_batch_stats {
 
 int pcl_render_batch(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, const pcl_wf_config* cfg,
                      pcl_batch_stats* stats);
-/* Times candidate configurations on this job (coordinate descent over the
- * knobs) and returns the fastest; also remembered as the context default. */
+/* Times candidate configurations on this job (coordinate descent over slots,
+ * blocks_per_cu, stack_k, iters_per_graph and inline_shadows) and returns the
+ * fastest; also remembered as the context default.  drain_below and streams
+ * stay 0 (the rules above, applied to the job being rendered): their best
+ * values grow with the samples per pixel, which a short tuning job cannot show. */
 int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf_config* best,
                     double* best_ms);
 

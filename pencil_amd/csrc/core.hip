@@ -115,8 +115,10 @@ struct DeferBufs {
 
 // The per-pixel body shared by both grids: render_pixel, or with DEFER its
 // deferred-glass form (path tracing only), which leaves the pixel to k_sum
-// when it deferred a sample.
-template <int PT, bool DEFER, class LC>
+// when it deferred a sample.  DEFER: 0 = in place, 1 = deferred (Whitted
+// draws nothing: hard lights), 2 = deferred with the subtree walk (soft
+// lights).
+template <int PT, int DEFER, class LC>
 __device__ __forceinline__ void shade_pixel(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j,
                                             Frame* F, Stack& stk, LC& c, float* out, DeferBufs& db) {
     const int64_t p = (int64_t)j * st.width + i;
@@ -124,8 +126,8 @@ __device__ __forceinline__ void shade_pixel(const DevScene& S, const CamRec& C, 
     if (DEFER) {
         int first;
         db.sink.pixel = (int32_t)p;
-        if (!render_pixel_defer(S, C, st, i, j, F, stk, c, db.sink, db.samp + p * 3, (size_t)db.n_px * 3, px,
-                                first)) {
+        if (!render_pixel_defer<DEFER == 2>(S, C, st, i, j, F, stk, c, db.sink, db.samp + p * 3,
+                                            (size_t)db.n_px * 3, px, first)) {
             float* q = db.prefix + p * 3;
             q[0] = px.x; q[1] = px.y; q[2] = px.z;
             db.first[p] = first;
@@ -144,7 +146,7 @@ __device__ __forceinline__ void shade_pixel(const DevScene& S, const CamRec& C, 
 // per SIMD (faster than 183 / 2 waves; 128 / 4 waves is no faster); path
 // tracing 128 VGPRs, 4 waves (the 1M-triangle Cornell frame 1.19x faster
 // than at 168 / 3 waves, and faster than 96 / 5 waves).
-template <bool COUNT, int PT, bool DEFER>
+template <bool COUNT, int PT, int DEFER>
 __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render(DevScene S, CamRec C, RSettings st, float* __restrict__ out,
                                                    unsigned long long* ctr, DeferBufs db) {
     __shared__ unsigned long long lds[kStackK * kBlock];
@@ -249,7 +251,7 @@ struct ChunkFetch {
 // k_render with a persistent grid: same per-pixel program, pixels in 64-pixel
 // row segments per wave; a wave takes its next segment as soon as its own
 // lanes are done, independent of the other wave of its block.
-template <bool COUNT, int PT, bool DEFER>
+template <bool COUNT, int PT, int DEFER>
 __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScene S, CamRec C, RSettings st,
                                                                      float* __restrict__ out, unsigned long long* ctr,
                                                                      uint32_t* heads, int n_chunks, DeferBufs db) {
@@ -879,10 +881,13 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
         db.first = ctx->d_first;
         db.n_px = (int64_t)st.width * st.height;
     }
-    auto pkern = mode == 1 ? (defer ? (counts ? k_render_p<true, 1, true> : k_render_p<false, 1, true>)
-                                    : (counts ? k_render_p<true, 1, false> : k_render_p<false, 1, false>))
-               : mode == 2 ? (counts ? k_render_p<true, 2, false> : k_render_p<false, 2, false>)
-                           : (counts ? k_render_p<true, 0, false> : k_render_p<false, 0, false>);
+    // deferred with the subtree walk only where Whitted draws (soft lights)
+    const bool walk = defer && db.sink.draws > 0;
+    auto pkern = mode == 1 ? (walk ? (counts ? k_render_p<true, 1, 2> : k_render_p<false, 1, 2>)
+                              : defer ? (counts ? k_render_p<true, 1, 1> : k_render_p<false, 1, 1>)
+                                      : (counts ? k_render_p<true, 1, 0> : k_render_p<false, 1, 0>))
+               : mode == 2 ? (counts ? k_render_p<true, 2, 0> : k_render_p<false, 2, 0>)
+                           : (counts ? k_render_p<true, 0, 0> : k_render_p<false, 0, 0>);
     int& bpc = ctx->render_bpc[mode];
     if (bpc <= 0) {
         int nb = 0;
@@ -906,10 +911,11 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
         hipLaunchKernelGGL(pkern, dim3(pgrid), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr,
                            ctx->d_heads, n_chunks, db);
     } else {
-        auto kern = mode == 1 ? (defer ? (counts ? k_render<true, 1, true> : k_render<false, 1, true>)
-                                       : (counts ? k_render<true, 1, false> : k_render<false, 1, false>))
-                  : mode == 2 ? (counts ? k_render<true, 2, false> : k_render<false, 2, false>)
-                              : (counts ? k_render<true, 0, false> : k_render<false, 0, false>);
+        auto kern = mode == 1 ? (walk ? (counts ? k_render<true, 1, 2> : k_render<false, 1, 2>)
+                                 : defer ? (counts ? k_render<true, 1, 1> : k_render<false, 1, 1>)
+                                         : (counts ? k_render<true, 1, 0> : k_render<false, 1, 0>))
+                  : mode == 2 ? (counts ? k_render<true, 2, 0> : k_render<false, 2, 0>)
+                              : (counts ? k_render<true, 0, 0> : k_render<false, 0, 0>);
         hipLaunchKernelGGL(kern, dim3(tiles), dim3(kBlock), 0, ctx->stream, sl->dev, C, st, d_out_rgb, ctx->d_ctr, db);
     }
     HIP_TRY(ctx, hipGetLastError());
@@ -995,6 +1001,7 @@ static int batch_prepare(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, s
         r.out = b.d_out_rgb;
         r.scene = i;
         r.glass = sl->has_glass ? 1 : 0;
+        r.draws = whitted_draws_per_node(*sl, r.st);
         r.pix_begin = begin;
         begin += pcl::wf::padded_pixels(b.settings.width, b.settings.height);
         its.push_back(r);
@@ -1082,21 +1089,23 @@ int pcl_wf_autotune(pcl_ctx* ctx, const pcl_batch_item* items, int32_t n, pcl_wf
     for (int64_t v : {(int64_t)1 << 18, (int64_t)1 << 19, (int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22})
         if (v < total) slot_opts.push_back((int)v);
     slot_opts.push_back((int)std::min<int64_t>(total, 1 << 23));
-    // The drain threshold is not tuned: the tail it cuts grows with the
-    // samples per pixel, so a short tuning job cannot rank the candidates (at
-    // 2x2 spp it picked 48 waves' worth of lanes per CU, 3% slower at 256 spp
-    // than the rule, the traversal grids' lanes).  Shadow rays per vertex
-    // traced inline: none, one, or up to 4 / 32.
-    // Concurrent engines: one, two or three.
+    // The drain threshold and the concurrent engines are not tuned: the tail
+    // they cut grows with the samples per pixel, so a short tuning job cannot
+    // rank the candidates.  At 2x2 spp the tuner picked 48 waves' worth of
+    // lanes per CU for the drain, 3% slower at 256 spp than the rule (the
+    // traversal grids' lanes), and two engines, 6% slower at 256 spp than the
+    // rule's three (Cornell + mesh: 2.65 s vs 2.50 s); both stay 0 = the
+    // rule, applied to the job being rendered.  Shadow rays per vertex traced
+    // inline: none, one, or up to 4 / 32.
     const std::vector<int> inline_opts{-1, 1, 4, 32};
-    for (int knob = 0; knob < 6; ++knob) {
+    for (int knob = 0; knob < 5; ++knob) {
         std::vector<int> opts = knob == 0 ? slot_opts : knob == 1 ? std::vector<int>{4, 6, 10, 12, 16}
                               : knob == 2 ? std::vector<int>{8, 16, 32} : knob == 3 ? std::vector<int>{1, 4, 16}
-                              : knob == 4 ? inline_opts : std::vector<int>{1, 2, 3};
+                              : inline_opts;
         for (int v : opts) {
             pcl::wf::Config c = cur;
             (knob == 0 ? c.slots : knob == 1 ? c.blocks_per_cu : knob == 2 ? c.stack_k
-                       : knob == 3 ? c.iters_per_graph : knob == 4 ? c.inline_shadows : c.streams) = v;
+                       : knob == 3 ? c.iters_per_graph : c.inline_shadows) = v;
             float ms;
             if ((rc = time_cfg(c, ms))) return fail(ctx, rc, "autotune: " + err);
             if (ms < cur_ms) { cur_ms = ms; cur = c; }

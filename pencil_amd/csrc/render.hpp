@@ -475,7 +475,7 @@ PCL_D int whitted_shaded_nodes(const DevScene& S, const RSettings& st, Ray r, in
 // loop (its tangent frame is the diffuse branch's, computed earlier).
 // `sink`: where a glass vertex may be deferred (GlassSink); P.deferred is
 // then set and P.R is not the sample's value.
-template <class LC, class SINK = NoSink>
+template <class LC, class SINK = NoSink, bool WALK = false>
 PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& rng, Frame* F, Stack& stk, LC& c,
                      SINK* sink = nullptr) {
     const bool mis = st.pathtrace == 2;
@@ -526,7 +526,9 @@ PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& 
         int nf = 0;
         Ray rr = push_glass(F, nf, P.cur, rec, albedo, ior, P.depth, st.shadow_bias);
         if (sink && sink->append(F[0], rr, P.T, P.R, emission, rng.s)) {
-            if (sink->draws) rng.skip((uint32_t)(sink->draws * whitted_shaded_nodes(S, st, rr, P.depth - 1, F, nf, stk)));
+            if constexpr (WALK) {
+                if (sink->draws) rng.skip((uint32_t)(sink->draws * whitted_shaded_nodes(S, st, rr, P.depth - 1, F, nf, stk)));
+            }
             P.deferred = true;
             return false;
         }
@@ -596,8 +598,12 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
 // deferred ones being filled in by the glass launch; a last launch adds them
 // to acc in sample order, so the pixel gets render_pixel's additions in
 // render_pixel's order.  Returns true, with the pixel value in px, when no
-// sample was deferred (first = n^2).
-template <class LC>
+// sample was deferred (first = n^2).  WALK: the scene's Whitted draws (soft
+// lights), so a deferred subtree's closest hits are walked to skip them; a
+// program without the walk is compiled for scenes whose Whitted draws nothing
+// (the walk's code alone, never run, costs the megakernel 8 % on the hard-light
+// Cornell + mesh frame).
+template <bool WALK, class LC>
 PCL_D bool render_samples_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, int k0,
                                 Rng& rng, f3 acc, Frame* F, Stack& stk, LC& c, GlassSink& sink, float* samp,
                                 size_t stride, f3& px, int& first) {
@@ -615,7 +621,7 @@ PCL_D bool render_samples_defer(const DevScene& S, const CamRec& C, const RSetti
         P.cur = r; P.T = f3(1, 1, 1); P.R = f3(0, 0, 0); P.depth = st.bounces;
         sink.k = k;
         while (P.depth > 0) {
-            if (!path_step(S, st, P, rng, F, stk, c, &sink)) break;
+            if (!path_step<LC, GlassSink, WALK>(S, st, P, rng, F, stk, c, &sink)) break;
         }
         if (P.deferred) {
             if (first == n * n) first = k;
@@ -631,22 +637,12 @@ PCL_D bool render_samples_defer(const DevScene& S, const CamRec& C, const RSetti
 }
 
 // The whole pixel (render_pixel's generator seed, samples 0..n^2-1).
-template <class LC>
+template <bool WALK, class LC>
 PCL_D bool render_pixel_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F,
                               Stack& stk, LC& c, GlassSink& sink, float* samp, size_t stride, f3& px, int& first) {
     Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
-    return render_samples_defer(S, C, st, i, j, 0, rng, f3(0, 0, 0), F, stk, c, sink, samp, stride, px, first);
-}
-
-// Generator draws of one shaded Whitted node (whitted_local): a unit_disc
-// (2 draws) per sample of every point light with a radius.
-PCL_D int whitted_draws(const DevScene& S, const RSettings& st) {
-    int d = 0;
-    for (int li = 0; li < S.n_lights; ++li) {
-        const LightRec L = load_light(S.lights, li);
-        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) d += 2 * st.soft_samples;
-    }
-    return d;
+    return render_samples_defer<WALK>(S, C, st, i, j, 0, rng, f3(0, 0, 0), F, stk, c, sink, samp, stride, px,
+                                      first);
 }
 
 // One deferred glass vertex (a GlassSink task, 8 float4 at q): the Whitted

@@ -834,9 +834,11 @@ struct FinishDefer {
 // as pixels that see glass -- then runs without a barrier per query.  With
 // DEFER (path-traced jobs) a path that hits glass leaves the vertex's Whitted
 // subtree to k_finish_glass, as the megakernel's deferred program does, and a
-// pixel that deferred a sample is left to k_finish_sum.
+// pixel that deferred a sample is left to k_finish_sum; DEFER 2 compiles the
+// subtree walk that skips the generator past Whitted's draws (an item with
+// soft lights), 1 leaves it out.
 // Register budget as the megakernel's for the same mode.
-template <int PT, bool DEFER>
+template <int PT, int DEFER>
 __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, const int4* meta, const float4* accs,
                                                                     int from, int to, FinishDefer fd) {
     __shared__ unsigned long long lds[kStackK * kWfBlock];
@@ -855,15 +857,15 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
         Rng rng((uint32_t)m.w);
         f3 px;
         bool done = true;
-        if constexpr (DEFER) {
+        if constexpr (DEFER > 0) {
             const int64_t p = e - from;
             GlassSink sink = fd.sink;
-            sink.draws = whitted_draws(D, it.st);
+            sink.draws = it.draws;
             sink.pixel = (int32_t)p;
             sink.item = m.x;
             int first;
-            done = render_samples_defer(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F, stk, c, sink,
-                                        fd.samp + p * 3, (size_t)fd.n_px * 3, px, first);
+            done = render_samples_defer<DEFER == 2>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F,
+                                                    stk, c, sink, fd.samp + p * 3, (size_t)fd.n_px * 3, px, first);
             fd.first[p] = first;
             if (!done) {
                 float* q = fd.prefix + p * 3;
@@ -986,15 +988,17 @@ void Engine::release() {
 }
 
 // The finish launch's deferred-glass buffers: 12 B per listed pixel and
-// sample, a task array sized for half of those samples (the listed pixels are
-// the job's longest, often glass-heavy; tasks beyond it are evaluated in
-// place), 16 B per listed pixel; together under half of the device's free
-// memory.  The task counter is 32-bit, so n_px * nn stays below 2^32.
+// sample, a task array sized for a quarter of those samples (the listed
+// pixels are the job's longest, often glass-heavy: 30 glass vertices per
+// listed pixel on the 256-spp Cornell + mesh frame, against a capacity of 64;
+// tasks beyond it are evaluated in place), 16 B per listed pixel; together
+// under half of the device's free memory.  The task counter is 32-bit, so
+// n_px * nn stays below 2^32.
 bool Engine::defer_buffers(int64_t n_px, int64_t nn) {
     if (n_px <= 0 || n_px * nn >= ((int64_t)1 << 32)) return false;
     const size_t samp = (size_t)(n_px * nn * 3);
     const size_t tasks = defer_tasks > 0 ? (size_t)defer_tasks
-                                         : std::min<size_t>((size_t)(n_px * nn / 2 + 4096), (size_t)1 << 27);
+                                         : std::min<size_t>((size_t)(n_px * nn / 4 + 4096), (size_t)1 << 27);
     size_t need = 0;
     if (samp > samp_cap) need += samp * sizeof(float);
     if (tasks > gtask_cap) need += tasks * 8 * sizeof(float4);
@@ -1145,10 +1149,11 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     bool any_path = false, any_whitted = false;
     for (const auto& it : its) (it.st.pathtrace ? any_path : any_whitted) = true;
     // finish kernel over records [from .. to) on stream st
-    bool any_glass = false;
+    bool any_glass = false, any_draws = false;
     int64_t nn_max = 1;
     for (const auto& it : its) {
         any_glass |= it.glass != 0;
+        any_draws |= it.draws > 0;
         nn_max = std::max<int64_t>(nn_max, (int64_t)it.st.samples * it.st.samples);
     }
     // Deferred glass vertices in the finish launch (as the megakernel's): for
@@ -1169,7 +1174,8 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
             fd.prefix = d_prefix;
             fd.first = d_first;
             fd.n_px = n_px;
-            hipLaunchKernelGGL((k_finish<1, true>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+            if (any_draws) hipLaunchKernelGGL((k_finish<1, 2>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+            else hipLaunchKernelGGL((k_finish<1, 1>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
             int nb = 0;
             hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_finish_glass, kWfBlock, 0);
             if (e != hipSuccess) return e;
@@ -1180,9 +1186,9 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
             if ((e = hipGetLastError()) != hipSuccess) return e;
             return hipMemcpyAsync(&h_pending[1], d_gcount, sizeof(uint32_t), hipMemcpyDeviceToHost, st);
         }
-        if (any_path && !any_whitted) hipLaunchKernelGGL((k_finish<1, false>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
-        else if (!any_path) hipLaunchKernelGGL((k_finish<0, false>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
-        else hipLaunchKernelGGL((k_finish<-1, false>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+        if (any_path && !any_whitted) hipLaunchKernelGGL((k_finish<1, 0>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+        else if (!any_path) hipLaunchKernelGGL((k_finish<0, 0>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+        else hipLaunchKernelGGL((k_finish<-1, 0>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
         return hipGetLastError();
     };
     // timing events, destroyed on every return path

@@ -25,6 +25,8 @@ struct ItemRec {
     int32_t scene;
     int32_t glass;       // the scene has a dielectric material
     int64_t pix_begin;   // first job pixel index of this item
+    int32_t draws;       // generator draws per shaded Whitted node (soft lights)
+    int32_t pad;
 };
 
 // Pixel-slot state, structure of arrays of 16-byte records (one per slot),

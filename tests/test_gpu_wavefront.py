@@ -199,6 +199,7 @@ def test_autotune_returns_a_working_config(core):
     try:
         best, ms = core.autotune([(sid, hs.camera(), hs.settings(), d)])
         assert best.stack_k in (8, 16, 32) and best.slots > 0 and ms > 0
+        assert best.drain_below == 0 and best.streams == 0   # left to the rules
         mk, _ = core.render(sid, hs.camera(), hs.settings())
         wf, _ = core.render_wavefront(sid, hs.camera(), hs.settings(), best)
         assert np.array_equal(wf, mk)
