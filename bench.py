@@ -383,22 +383,23 @@ def leg_glass_defer(core, pa, args, threads, model):
                 desc.lights[i].radius = 0.0
         W, H = st.width, st.height
         res = {}
-        for name, env in (("in_place", "0"), ("deferred", "1")):
-            os.environ["PCL_DEFER_GLASS"] = env
-            c = pa.Core(0)
-            try:
-                sid = c.upload(desc)
-                d = c.alloc(W * H * 12)
+        # the bench's own context, toggled: a second context in the process
+        # would take a hardware queue from the batched tracer's three streams
+        sid = core.upload(desc)
+        d = core.alloc(W * H * 12)
+        try:
+            for name, enable in (("in_place", False), ("deferred", True)):
+                core.set_glass_deferral(enable)
                 r = {}
                 for path in ("plain", "batched"):
-                    run = (lambda: c.render_device(sid, cam, st, d)) if path == "plain" else \
-                        (lambda: c.render_batch([(sid, cam, st, d)], cfg))
+                    run = (lambda: core.render_device(sid, cam, st, d)) if path == "plain" else \
+                        (lambda: core.render_batch([(sid, cam, st, d)], cfg))
                     run()   # warm-up (buffers, graph)
                     t0 = time.perf_counter()
                     s = run()
                     ms = (time.perf_counter() - t0) * 1000.0
                     img = np.empty((H, W, 3), dtype=np.float32)
-                    c.d2h(img, d)
+                    core.d2h(img, d)
                     imgs.append(img)
                     q = s.closest_queries + s.anyhit_queries
                     r[path] = {"ms": ms, "kernel_ms": s.kernel_ms, "mrays_s": q / ms / 1e3, "queries": q,
@@ -406,11 +407,10 @@ def leg_glass_defer(core, pa, args, threads, model):
                     log(f"glass_defer {light} {path} {name}: {ms:.1f} ms, {q / ms / 1e3:.1f} Mrays/s, "
                         f"{s.deferred_glass} deferred")
                 res[name] = r
-                c.free(d)
-                c.free_scene(sid)
-            finally:
-                c.close()
-        os.environ.pop("PCL_DEFER_GLASS", None)
+        finally:
+            core.set_glass_deferral(True)
+            core.free(d)
+            core.free_scene(sid)
         res["speedup_deferred_plain"] = res["in_place"]["plain"]["ms"] / res["deferred"]["plain"]["ms"]
         res["speedup_deferred_batched"] = res["in_place"]["batched"]["ms"] / res["deferred"]["batched"]["ms"]
         out[light] = res

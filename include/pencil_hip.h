@@ -234,6 +234,15 @@ int pcl_render_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam,
 int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                       float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts);
 
+/* Deferred glass vertices of path-traced renders (pcl_render*, and the batched
+ * tracer's finish launch): enable = 0 evaluates every glass vertex in place,
+ * otherwise they go to a compacted second launch when the buffers fit; the
+ * image is the same bit for bit either way.  max_tasks > 0 caps the task array
+ * (vertices beyond it are evaluated in place), 0 sizes it by the frame.
+ * Starts from the environment (PCL_DEFER_GLASS, PCL_DEFER_TASKS), else on and
+ * automatic.  No reference counterpart (a device scheduling choice). */
+int pcl_set_glass_deferral(pcl_ctx* ctx, int32_t enable, int64_t max_tasks);
+
 /* ---- batched wavefront tracer --------------------------------------------
  * Renders many (scene, camera, settings) items -- scenes of uneven size and
  * resolution -- in one job on a persistent wavefront tracer: a fixed pool of

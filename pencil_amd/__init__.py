@@ -140,6 +140,7 @@ def hip_lib():
              [_P, C.c_int32, C.POINTER(Camera), C.POINTER(Settings), _P, C.POINTER(Stats), C.POINTER(WorkCounts)])
         _sig(lib, "pcl_render_batch", C.c_int,
              [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(BatchStats)])
+        _sig(lib, "pcl_set_glass_deferral", C.c_int, [_P, C.c_int32, C.c_int64])
         _sig(lib, "pcl_wf_autotune", C.c_int,
              [_P, C.POINTER(BatchItem), C.c_int32, C.POINTER(WfConfig), C.POINTER(C.c_double)])
         _sig(lib, "pcl_scene_set_materials", C.c_int, [_P, C.c_int32, C.POINTER(Material), C.c_int32])
@@ -502,6 +503,11 @@ class Core:
         self._check(self.lib.pcl_render_counts(self.ctx, sid, C.byref(cam), C.byref(settings), C.c_void_p(d_out),
                                                C.byref(st), C.byref(wc)), "pcl_render_counts")
         return st, wc
+
+    def set_glass_deferral(self, enable: bool, max_tasks: int = 0):
+        """Glass vertices of path-traced renders deferred (the default) or in place."""
+        self._check(self.lib.pcl_set_glass_deferral(self.ctx, int(bool(enable)), int(max_tasks)),
+                    "pcl_set_glass_deferral")
 
     def render_batch(self, jobs, cfg: WfConfig = None):
         """jobs: list of (scene id, Camera, Settings, device output pointer)."""

@@ -949,6 +949,16 @@ int pcl_render_device(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const 
     return render_impl(ctx, scene, cam, settings, d_out_rgb, stats, nullptr);
 } PCL_CATCH(ctx)
 
+int pcl_set_glass_deferral(pcl_ctx* ctx, int32_t enable, int64_t max_tasks) {
+    if (!ctx) return PCL_ERR_INVALID;
+    if (max_tasks < 0) return fail(ctx, PCL_ERR_INVALID, "max_tasks must be >= 0");
+    ctx->defer_glass = enable ? 1 : 0;
+    ctx->defer_tasks = max_tasks;
+    ctx->wf->defer_glass = ctx->defer_glass;
+    ctx->wf->defer_tasks = ctx->defer_tasks;
+    return PCL_OK;
+}
+
 int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
                       float* d_out_rgb, pcl_stats* stats, pcl_work_counts* counts) try {
     pcl_work_counts tmp;

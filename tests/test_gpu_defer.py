@@ -234,3 +234,36 @@ def test_batched_mixed_job_finishes_in_place(cores):
     for img, ref in zip(imgs, refs):
         assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
     assert bs.finished_pixels > 0 and bs.deferred_glass == 0
+
+
+def test_runtime_toggle_equals_the_environment():
+    """pcl_set_glass_deferral on one context: off, on, on with a 37-task cap,
+    for plain and batched renders; images bit-identical, counts as set; a
+    negative cap is refused."""
+    hs, st = _scene("cornell_c5", 96, 72, 3, 1, 8, False)
+    core = pa.Core(0)
+    try:
+        sid = core.upload(hs.desc())
+        d = core.alloc(st.width * st.height * 12)
+        cfg = pa.WfConfig(slots=6000, blocks_per_cu=2, stack_k=16, iters_per_graph=1, drain_below=1 << 30,
+                          streams=1)
+        imgs, counts = [], []
+        for enable, cap in ((False, 0), (True, 0), (True, 37)):
+            core.set_glass_deferral(enable, cap)
+            img, s = core.render(sid, hs.camera(), st)
+            bs = core.render_batch([(sid, hs.camera(), st, d)], cfg)
+            bimg = np.empty_like(img)
+            core.d2h(bimg, d)
+            imgs += [img, bimg]
+            counts.append((s.deferred_glass, bs.deferred_glass))
+        for im in imgs[1:]:
+            assert np.array_equal(im.view(np.uint32), imgs[0].view(np.uint32))
+        assert counts[0] == (0, 0)
+        assert counts[1][0] > 37 and counts[1][1] > 37
+        assert counts[2] == (37, 37)
+        with pytest.raises(pa.CoreError):
+            core.set_glass_deferral(True, -1)
+        core.free(d)
+        core.free_scene(sid)
+    finally:
+        core.close()
