@@ -173,7 +173,7 @@ def live_traffic(batched_cfg):
         for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
             rows = _pmc_run(ctr, args, tmp, "head_" + ctr)
             plain = sorted({int(r["Dispatch_Id"]): float(r["Counter_Value"]) for r in rows
-                            if "k_render_p<false, 2, false>" in r["Kernel_Name"]}.items())[:2]
+                            if "k_render_p<false, 2, 0>" in r["Kernel_Name"]}.items())[:2]
             wf = [float(r["Counter_Value"]) for r in rows if "pcl::wf::" in r["Kernel_Name"]]
             if len(plain) != 2 or not wf:
                 raise RuntimeError(f"unexpected dispatches under rocprofv3 ({len(plain)} plain)")

@@ -62,7 +62,8 @@ enum : int {
     V_P_MISS = 4,      // path miss: R += T * bg, sample ends
     V_P_BOUNCE = 5,    // path bounce: continue with the new ray
     V_P_END = 6,       // path ends after this vertex (Russian roulette)
-    V_P_GLASS = 7      // path glass: Whitted subtree from the pushed frame
+    V_P_GLASS = 7,     // path glass: Whitted subtree from the pushed frame
+    V_DONE = 8         // pixel finished and written by the closest-hit kernel (FOLD)
 };
 
 // ---- wave-level queue helpers --------------------------------------------
@@ -200,6 +201,10 @@ __device__ __forceinline__ void load_lane(const Slots& S, int s, Lane& L) {
     L.mode = c.x & 15; L.glass_root = (c.x >> 4) & 1; L.outcome_aux = (c.x >> 5) & 1; L.outcome = c.x >> 8;
     L.depth = c.y; L.nf = c.z; L.nsh = c.w;
     L.ray_set = false;
+    if (L.outcome == V_DONE) {   // the slot only wants a new pixel
+        L.acc = L.T = L.R = f3(0, 0, 0);
+        return;
+    }
     L.acc = xyz(S.acc[s]); L.T = xyz(S.T[s]); L.R = xyz(S.R[s]);
 }
 
@@ -424,8 +429,13 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // One wave's chunk of vertices of stripe g (s = -1: lane without one).  Every
 // lane of the wave calls it: the shadow-queue offsets are allocated per wave.
 // ro/rd: the query ray with the closest-hit result in .w (t_max at
-// acceptance, hit slot).
-template <bool ONE, class ST>
+// acceptance, hit slot).  FOLD: a Whitted vertex that ends its pixel -- no
+// child ray, no frame waiting, the pixel's last sample, every shadow ray
+// traced here -- is folded here with resolve's operations in resolve's order
+// (fold_whitted_local, finish_sample) and the pixel written; its light
+// samples and material never go through the slot arrays, and k_resolve only
+// hands the slot a new pixel (V_DONE).
+template <bool ONE, bool FOLD, class ST>
 __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 meta, int4 ctl, float4 ro, float4 rd,
                                             ST& stk, uint32_t& n_inline) {
         const Slots& S = C.sl;
@@ -454,8 +464,14 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
         int outcome;
         int flags = ctl.x & 0xF0;        // keeps glass_root / outcome_aux
         int nf = ctl.z, ns = 0;
+        const bool last = FOLD && !path && nf == 0 && !(ctl.x & 0x10) && meta.z + 1 == st.samples * st.samples;
+        f3 result(0, 0, 0);   // a FOLDed vertex's value
         if (slot < 0) {
             outcome = path ? V_P_MISS : V_W_MISS;
+            if (last) {
+                result = ld3(D.bg);
+                outcome = V_DONE;
+            }
         } else {
             HitRec rec;
             PrimView P = load_prim(D.prims, slot);
@@ -464,7 +480,9 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
             int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
             load_mat(D.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
             Rng rng((uint32_t)meta.w);
-            S.alb[s] = f4(albedo);
+            const bool term = last && inl && !(kind == PCL_MAT_METAL && refl > 0.0f) && kind != PCL_MAT_DIELECTRIC;
+            if (!term) S.alb[s] = f4(albedo);
+            result = emission;
             float4* SH = S.sh + s;   // sample q at SH[q * S.sh_stride]
             uint8_t* OC = S.occl + s;
             const f3 T = path ? xyz(S.T[s]) : f3(1, 1, 1);
@@ -474,6 +492,7 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                 const LightRec L = load_light(D.lights, li);
                 const f3 lpos = ld3(L.pos), I = ld3(L.I);
                 const int nsamp = path ? 1 : ((L.radius > 0.0f) ? st.soft_samples : 1);
+                f3 contrib(0, 0, 0);
                 for (int q = 0; q < nsamp; ++q) {
                     f3 L_pos = lpos;
                     f3 Ld;
@@ -502,18 +521,19 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                             c += spec_color * I * (sp * spec);
                         }
                     }
-                    SH[ns * S.sh_stride] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8) | skip));
+                    if (!term) SH[ns * S.sh_stride] = make_float4(c.x, c.y, c.z, __int_as_float(li | (nsamp << 8) | skip));
+                    bool occ = false;
                     if (!L.emissive && inl) {
                         Ray sr;
                         sr.o = sh_o;
                         sr.d = Ld;
                         float tb, ta;
                         TravCount<false> tc;
-                        OC[ns * S.sh_stride] = traverse<true>(D, sr, st.shadow_bias, dist - st.shadow_bias, stk, tb, ta, tc) >= 0;
+                        occ = traverse<true>(D, sr, st.shadow_bias, dist - st.shadow_bias, stk, tb, ta, tc) >= 0;
                         ++n_inline;
-                    } else {
-                        OC[ns * S.sh_stride] = 0;
                     }
+                    if (!term) OC[ns * S.sh_stride] = occ;
+                    else if (!occ) contrib += c;
                     if (!L.emissive && !inl) {
                         C.q.sh_who[o] = make_int2(s, ns);
                         C.q.sh_ray[2 * o] = make_float4(sh_o.x, sh_o.y, sh_o.z, st.shadow_bias);
@@ -522,6 +542,7 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                     }
                     ns++;
                 }
+                if (term) result += contrib / (float)nsamp;
             }
             if (path) {
                 // radiance += T * emission comes before the NEE terms (:861)
@@ -571,6 +592,9 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                         S.NT[s] = f4(NT);
                     }
                 }
+            } else if (term) {
+                result += albedo * ld3(D.amb);
+                outcome = V_DONE;
             } else {
                 S.pend[s] = f4(emission);
                 outcome = V_W_LEAF;
@@ -594,6 +618,12 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
             meta.w = (int)rng.s;
             S.meta[s] = meta;
         }
+        if (outcome == V_DONE) {   // finish_sample of the pixel's last sample
+            const f3 acc = xyz(S.acc[s]) + result;
+            const f3 px = acc / (float)(st.samples * st.samples);
+            float* o = it.out + (size_t)meta.y * 3;
+            o[0] = px.x; o[1] = px.y; o[2] = px.z;
+        }
         S.ctl[s] = make_int4((ctl.x & 15) | flags | (outcome << 8), ctl.y, nf, ns);
 }
 
@@ -602,8 +632,8 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
 // hit never goes through memory and the slot state is read once.  ONE: the
 // job has a single item (one scene, camera and settings), so its records
 // are read with uniform (scalar) loads instead of per lane through the
-// slot's item index.
-template <int K, bool ONE>
+// slot's item index.  FOLD: see shade_chunk (jobs with inline shadow rays).
+template <int K, bool ONE, bool FOLD>
 __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
     if (C.q.ctr[C_PENDING] == 0) return;   // empty iteration
     __shared__ unsigned long long lds[K * kWfBlock];
@@ -634,7 +664,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
             ro.w = ta;
             rd.w = __int_as_float(slot);
         }
-        shade_chunk<ONE>(C, g, s, meta, ctl, ro, rd, st, n_inline);
+        shade_chunk<ONE, FOLD>(C, g, s, meta, ctl, ro, rd, st, n_inline);
     }
     wave_add_u64(n_inline, &C.q.stats[(1 + home) * kStatLine + ST_ANY]);
 }
@@ -684,6 +714,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
             int status;
             f3 result(0, 0, 0);
             switch (L.outcome) {
+            case V_DONE: status = ADV_NEED_PIXEL; break;   // written by the closest-hit kernel
             case V_W_MISS: status = 3; result = ld3(D.bg); break;
             case V_W_LEAF: status = 3; result = fold_whitted_local(S, s, L.nsh, xyz(S.pend[s]), xyz(S.alb[s]), amb); break;
             case V_W_CHILD: {
@@ -1063,18 +1094,25 @@ int Engine::ensure(int slots, int items, std::string& err) {
     return PCL_OK;
 }
 
-template <int K, bool ONE>
+template <int K, bool ONE, bool FOLD>
 static void launch_iter(const Ctx& c, int grid_trav, int grid_other, hipStream_t st) {
-    hipLaunchKernelGGL((k_closest_shade<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
+    hipLaunchKernelGGL((k_closest_shade<K, ONE, FOLD>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL((k_anyhit<K, ONE>), dim3(grid_trav), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_resolve<ONE>, dim3(grid_other), dim3(kWfBlock), 0, st, c);
     hipLaunchKernelGGL(k_swap, dim3(1), dim3(kStripes), 0, st, c.q);
 }
 
+// FOLD (Engine::run decides): the closest-hit kernel finishes terminal
+// one-sample Whitted pixels itself.
 template <int K>
-static void launch_iter(const Ctx& c, bool one, int grid_trav, int grid_other, hipStream_t st) {
-    if (one) launch_iter<K, true>(c, grid_trav, grid_other, st);
-    else launch_iter<K, false>(c, grid_trav, grid_other, st);
+static void launch_iter(const Ctx& c, bool one, bool fold, int grid_trav, int grid_other, hipStream_t st) {
+    if (one) {
+        if (fold) launch_iter<K, true, true>(c, grid_trav, grid_other, st);
+        else launch_iter<K, true, false>(c, grid_trav, grid_other, st);
+    } else {
+        if (fold) launch_iter<K, false, true>(c, grid_trav, grid_other, st);
+        else launch_iter<K, false, false>(c, grid_trav, grid_other, st);
+    }
 }
 
 int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std::vector<DevScene>& scenes,
@@ -1116,21 +1154,28 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     c.q.part = part;
     c.q.parts = parts;
     const bool one = its.size() == 1;   // part of the graph key through graph_items
+    // Finishing pixels in the closest-hit kernel pays where most pixels end
+    // at their first vertex: one-sample Whitted jobs (the 1-spp stress frame
+    // 2 % faster, 10 % less HBM traffic); with more samples only a pixel's
+    // last sample could, and the larger kernel cost the 16-spp 64-scene job
+    // 1 %.  Needs inline shadow rays.
+    bool fold = cfg.inline_shadows > 0;
+    for (const auto& it : its) fold &= !it.st.pathtrace && it.st.samples == 1;
     const int grid_trav = cus * cfg.blocks_per_cu;
     const int grid_other = cus * 8;   // 4 waves per SIMD, the register limit of shade / resolve
     const bool graph_ok = (graph_exec && graph_key.slots == cfg.slots && graph_key.blocks_per_cu == cfg.blocks_per_cu &&
                            graph_key.stack_k == cfg.stack_k && graph_key.iters_per_graph == cfg.iters_per_graph &&
                            graph_key.inline_shadows == cfg.inline_shadows &&
                            graph_items == (int)its.size() && graph_total == total && graph_part == part &&
-                           graph_parts == parts);
+                           graph_parts == parts && graph_fold == fold);
     if (!graph_ok) {
         if (graph_exec) { (void)hipGraphExecDestroy(graph_exec); graph_exec = nullptr; }
         hipGraph_t g = nullptr;
         WF_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < cfg.iters_per_graph; ++i) {
-            if (cfg.stack_k == 8) launch_iter<8>(c, one, grid_trav, grid_other, stream);
-            else if (cfg.stack_k == 32) launch_iter<32>(c, one, grid_trav, grid_other, stream);
-            else launch_iter<16>(c, one, grid_trav, grid_other, stream);
+            if (cfg.stack_k == 8) launch_iter<8>(c, one, fold, grid_trav, grid_other, stream);
+            else if (cfg.stack_k == 32) launch_iter<32>(c, one, fold, grid_trav, grid_other, stream);
+            else launch_iter<16>(c, one, fold, grid_trav, grid_other, stream);
         }
         WF_TRY(hipStreamEndCapture(stream, &g));
         hipError_t e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
@@ -1141,6 +1186,7 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         graph_total = total;
         graph_part = part;
         graph_parts = parts;
+        graph_fold = fold;
     }
     if (!h_pending)
         WF_TRY(hipHostMalloc((void**)&h_pending, 2 * kStatusWords * sizeof(uint32_t), hipHostMallocDefault));

@@ -141,6 +141,7 @@ struct Engine {
     int graph_items = 0;
     int64_t graph_total = 0;
     int graph_part = -1, graph_parts = 0;
+    bool graph_fold = false;
     uint32_t* h_pending = nullptr;      // pinned, kStatusWords per graph launch in flight
     hipEvent_t ev_pending[2] = {nullptr, nullptr};
     // Deferred glass vertices of the finish launch (path-traced jobs): the

@@ -192,6 +192,53 @@ def test_batch_of_uneven_scenes_equals_individual_renders(core):
             core.free_scene(sid)
 
 
+@pytest.mark.parametrize("inline", [1, 4, 32])
+@pytest.mark.parametrize("depth", [1, 6])
+def test_pixels_finished_in_the_closest_hit_kernel(core, inline, depth):
+    """In one-sample Whitted jobs with inline shadow rays the closest-hit
+    kernel finishes a pixel whose vertex ends it (no child, no frame waiting,
+    every shadow ray inline) and writes it out; the others go through
+    k_resolve.  Several scenes in one job (hard and soft lights, metal, glass,
+    DOF, a stress mesh) and the stress scene alone, depth 1 and 6: images and
+    counts are the megakernel's."""
+    jobs, refs, bufs, sids, mstats = [], [], [], [], []
+    specs = [("three_spheres", dict(width=40, height=30, samples=1)), ("hard_shadows", dict(width=36, height=20, samples=1)),
+             ("soft_shadows", dict(width=30, height=24, samples=1)), ("dof", dict(width=24, height=24, samples=1)),
+             ("shapes", dict(width=33, height=17, samples=1))]
+    scenes = [pa.HostScene.from_pecl(golden(n + ".pecl"), **kw) for n, kw in specs]
+    scenes.append(pa.HostScene.stress(freq=30, n_spheres=200, seed=5, width=96, height=54))
+    try:
+        for hs in scenes:
+            s = hs.settings()
+            s.max_depth = depth
+            sid = core.upload(hs.desc())
+            sids.append(sid)
+            img, mst = core.render(sid, hs.camera(), s)
+            d = core.alloc(s.width * s.height * 12)
+            bufs.append((d, s))
+            jobs.append((sid, hs.camera(), s, d))
+            refs.append(img)
+            mstats.append(mst)
+        for one in (False, True):
+            sel = range(len(jobs)) if not one else [len(jobs) - 1]
+            cfg = pa.WfConfig(slots=20000, blocks_per_cu=2, stack_k=16, iters_per_graph=1, inline_shadows=inline)
+            bs = core.render_batch([jobs[i] for i in sel], cfg)
+            assert bs.used.inline_shadows == inline
+            assert bs.rays_traced == sum(mstats[i].rays_traced for i in sel)
+            assert bs.closest_queries == sum(mstats[i].closest_queries for i in sel)
+            assert bs.anyhit_queries == sum(mstats[i].anyhit_queries for i in sel)
+            for i in sel:
+                d, s = bufs[i]
+                out = np.empty((s.height, s.width, 3), dtype=np.float32)
+                core.d2h(out, d)
+                assert np.array_equal(out.view(np.uint32), refs[i].view(np.uint32)), specs[i][0] if i < len(specs) else "stress"
+    finally:
+        for d, _ in bufs:
+            core.free(d)
+        for sid in sids:
+            core.free_scene(sid)
+
+
 def test_autotune_returns_a_working_config(core):
     hs = pa.HostScene.from_pecl(golden("shapes.pecl"), width=96, height=72, samples=1)
     sid = core.upload(hs.desc())
