@@ -776,10 +776,21 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
     wave_add_u64(rays, &C.q.stats[(1 + home) * kStatLine + ST_RAYS]);
 }
 
-// Starts the job: every slot takes a pixel and its first query.  Slot wave
-// b (slots 64b .. 64b + 63) belongs to stripe b % kStripes.
-__global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
-    uint32_t rays = 0;
+// Starts the job and runs the closest-hit half of its first iteration: every
+// slot takes a pixel, makes its first ray (as k_resolve starts a pixel) and,
+// with the ray still in registers, traces and shades it as k_closest_shade
+// does; k_anyhit, k_resolve and k_swap then finish the iteration.  The slot
+// state is stored only where the shading or k_resolve reads it (meta, acc,
+// and T / R of a path), not the ray and control words the shading writes
+// over.  Slot wave b (slots 64b .. 64b + 63) belongs to stripe b % kStripes.
+// One lane per slot, the LDS stack of the default depth.
+template <bool ONE, bool FOLD>
+__global__ __launch_bounds__(kWfBlock, 4) void k_start_closest(Ctx C, int n_slots) {
+    __shared__ unsigned long long lds[kStackK * kWfBlock];
+    Stack st;
+    st.lds = (PCL_LDS unsigned long long*)(lds + threadIdx.x);
+    st.set_stride(kWfBlock);
+    uint32_t rays = 0, n_inline = 0;
     const int cur = C.q.ctr[C_CUR];
     const int s0 = wave_gid() * kWave;
     const int s = s0 + lane_id();
@@ -792,10 +803,37 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start(Ctx C, int n_slots) {
     L.ray_set = false;
     int a = (s < n_slots) ? ADV_NEED_PIXEL : ADV_IDLE;
     a = refill(C, s, L, a, pf, rays, false, true);
-    if (s < n_slots) store_lane(C.sl, s, L);
+    const Slots& S = C.sl;
+    if (s < n_slots) {
+        S.meta[s] = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
+        S.acc[s] = f4(L.acc);
+        if (L.mode == M_PATH) {
+            S.T[s] = f4(L.T);
+            S.R[s] = f4(L.R);
+        }
+        if (a != ADV_QUERY) S.ctl[s] = make_int4(pack_flags(L), L.depth, L.nf, L.nsh);
+    }
     const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + cur, g)]);
     if (pos >= 0) C.q.ray_q[cur][(size_t)g * C.q.qcap + pos] = s;
     wave_add_u64(rays, &C.q.stats[(1 + g) * kStatLine + ST_RAYS]);
+    // the first closest-hit query (k_closest_shade's body for this slot)
+    int4 meta = make_int4(0, 0, 0, 0), ctl = make_int4(0, 0, 0, 0);
+    float4 ro = make_float4(0, 0, 0, 0), rd = make_float4(0, 0, 0, 0);
+    if (a == ADV_QUERY) {
+        meta = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
+        ctl = make_int4(pack_flags(L), L.depth, L.nf, L.nsh);
+        const DevScene& D = ONE ? C.scenes[0] : C.scenes[C.items[L.item].scene];
+        Ray r;
+        r.o = L.o;
+        r.d = L.d;
+        float tb, ta;
+        TravCount<false> cnt;
+        const int slot = traverse<false>(D, r, 1e-4f, 1e9f, st, tb, ta, cnt);
+        ro = f4(L.o, ta);
+        rd = f4(L.d, __int_as_float(slot));
+    }
+    shade_chunk<ONE, FOLD>(C, g, a == ADV_QUERY ? s : -1, meta, ctl, ro, rd, st, n_inline);
+    wave_add_u64(n_inline, &C.q.stats[(1 + g) * kStatLine + ST_ANY]);
 }
 
 // End of an iteration, one lane per stripe: account the queries, flip the
@@ -1247,7 +1285,21 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
     WF_TRY(hipEventRecord(e0, stream));
     hipLaunchKernelGGL(k_reset, dim3(16), dim3(256), 0, stream, c.q);
-    hipLaunchKernelGGL(k_start, dim3((cfg.slots + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, stream, c, cfg.slots);
+    {   // the first iteration: start + closest hit in one kernel, then the rest of the iteration
+        const dim3 sg((cfg.slots + kWfBlock - 1) / kWfBlock), b(kWfBlock);
+        if (one) {
+            if (fold) hipLaunchKernelGGL((k_start_closest<true, true>), sg, b, 0, stream, c, cfg.slots);
+            else hipLaunchKernelGGL((k_start_closest<true, false>), sg, b, 0, stream, c, cfg.slots);
+            hipLaunchKernelGGL((k_anyhit<kStackK, true>), dim3(grid_trav), b, 0, stream, c);
+            hipLaunchKernelGGL(k_resolve<true>, dim3(grid_other), b, 0, stream, c);
+        } else {
+            if (fold) hipLaunchKernelGGL((k_start_closest<false, true>), sg, b, 0, stream, c, cfg.slots);
+            else hipLaunchKernelGGL((k_start_closest<false, false>), sg, b, 0, stream, c, cfg.slots);
+            hipLaunchKernelGGL((k_anyhit<kStackK, false>), dim3(grid_trav), b, 0, stream, c);
+            hipLaunchKernelGGL(k_resolve<false>, dim3(grid_other), b, 0, stream, c);
+        }
+        hipLaunchKernelGGL(k_swap, dim3(1), dim3(kStripes), 0, stream, c.q);
+    }
     WF_TRY(hipGetLastError());
     // Two graph launches in flight: while the host reads the status words
     // after launch k, launch k + 1 already runs, so the GPU never idles for
