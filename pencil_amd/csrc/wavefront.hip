@@ -434,10 +434,12 @@ __global__ __launch_bounds__(kWfBlock) void k_anyhit(Ctx C) {
 // traced here -- is folded here with resolve's operations in resolve's order
 // (fold_whitted_local, finish_sample) and the pixel written; its light
 // samples and material never go through the slot arrays, and k_resolve only
-// hands the slot a new pixel (V_DONE).
-template <bool ONE, bool FOLD, class ST>
-__device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 meta, int4 ctl, float4 ro, float4 rd,
-                                            ST& stk, uint32_t& n_inline) {
+// hands the slot a new pixel (V_DONE).  FRESH: the slot's first query of the
+// job (k_start_closest), whose sample sum is zero and whose meta record is
+// not in memory yet.  Returns the vertex outcome (V_NONE without a vertex).
+template <bool ONE, bool FOLD, bool FRESH, class ST>
+__device__ __forceinline__ int shade_chunk(const Ctx& C, int g, int s, int4 meta, int4 ctl, float4 ro, float4 rd,
+                                           ST& stk, uint32_t& n_inline) {
         const Slots& S = C.sl;
         const ItemRec* itp = ONE ? &C.items[0] : (s >= 0) ? &C.items[meta.x] : nullptr;
         const DevScene* Dp = ONE ? &C.scenes[0] : itp ? &C.scenes[itp->scene] : nullptr;
@@ -454,7 +456,7 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
         // after its closest hit; the others go through the shadow queue
         const bool inl = (int)n_test <= C.inline_max;
         size_t o = (size_t)g * C.q.shcap + wave_append_n(inl ? 0u : n_test, &C.q.ctr[sidx(S_SHADOW, g)]);
-        if (s < 0) return;
+        if (s < 0) return V_NONE;
         const ItemRec& it = *itp;
         const DevScene& D = *Dp;
         const RSettings& st = it.st;
@@ -472,6 +474,7 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
                 result = ld3(D.bg);
                 outcome = V_DONE;
             }
+            if (FRESH) S.meta[s] = meta;
         } else {
             HitRec rec;
             PrimView P = load_prim(D.prims, slot);
@@ -619,12 +622,13 @@ __device__ __forceinline__ void shade_chunk(const Ctx& C, int g, int s, int4 met
             S.meta[s] = meta;
         }
         if (outcome == V_DONE) {   // finish_sample of the pixel's last sample
-            const f3 acc = xyz(S.acc[s]) + result;
+            const f3 acc = (FRESH ? f3(0, 0, 0) : xyz(S.acc[s])) + result;
             const f3 px = acc / (float)(st.samples * st.samples);
             float* o = it.out + (size_t)meta.y * 3;
             o[0] = px.x; o[1] = px.y; o[2] = px.z;
         }
         S.ctl[s] = make_int4((ctl.x & 15) | flags | (outcome << 8), ctl.y, nf, ns);
+        return outcome;
 }
 
 // Closest hit and shading in one pass: the lane shades its vertex right
@@ -664,7 +668,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_closest_shade(Ctx C) {
             ro.w = ta;
             rd.w = __int_as_float(slot);
         }
-        shade_chunk<ONE, FOLD>(C, g, s, meta, ctl, ro, rd, st, n_inline);
+        shade_chunk<ONE, FOLD, false>(C, g, s, meta, ctl, ro, rd, st, n_inline);
     }
     wave_add_u64(n_inline, &C.q.stats[(1 + home) * kStatLine + ST_ANY]);
 }
@@ -780,9 +784,11 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_resolve(Ctx C) {
 // slot takes a pixel, makes its first ray (as k_resolve starts a pixel) and,
 // with the ray still in registers, traces and shades it as k_closest_shade
 // does; k_anyhit, k_resolve and k_swap then finish the iteration.  The slot
-// state is stored only where the shading or k_resolve reads it (meta, acc,
-// and T / R of a path), not the ray and control words the shading writes
-// over.  Slot wave b (slots 64b .. 64b + 63) belongs to stripe b % kStripes.
+// state is stored only where the shading or k_resolve reads it: T / R of a
+// path before the shading, which writes the meta record, the ray and the
+// control word itself; the (zero) sample sum after it, unless the shading
+// finished the pixel.  Slot wave b (slots 64b .. 64b + 63) belongs to stripe
+// b % kStripes.
 // One lane per slot, the LDS stack of the default depth.
 template <bool ONE, bool FOLD>
 __global__ __launch_bounds__(kWfBlock, 4) void k_start_closest(Ctx C, int n_slots) {
@@ -805,13 +811,15 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start_closest(Ctx C, int n_slot
     a = refill(C, s, L, a, pf, rays, false, true);
     const Slots& S = C.sl;
     if (s < n_slots) {
-        S.meta[s] = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
-        S.acc[s] = f4(L.acc);
         if (L.mode == M_PATH) {
             S.T[s] = f4(L.T);
             S.R[s] = f4(L.R);
         }
-        if (a != ADV_QUERY) S.ctl[s] = make_int4(pack_flags(L), L.depth, L.nf, L.nsh);
+        if (a != ADV_QUERY) {   // no query (an idle slot): the record as it is
+            S.meta[s] = make_int4(L.item, L.pixel, L.sample, (int)L.rng);
+            S.acc[s] = f4(L.acc);
+            S.ctl[s] = make_int4(pack_flags(L), L.depth, L.nf, L.nsh);
+        }
     }
     const int pos = wave_append(a == ADV_QUERY, &C.q.ctr[sidx(S_COUNT0 + cur, g)]);
     if (pos >= 0) C.q.ray_q[cur][(size_t)g * C.q.qcap + pos] = s;
@@ -832,7 +840,8 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_start_closest(Ctx C, int n_slot
         ro = f4(L.o, ta);
         rd = f4(L.d, __int_as_float(slot));
     }
-    shade_chunk<ONE, FOLD>(C, g, a == ADV_QUERY ? s : -1, meta, ctl, ro, rd, st, n_inline);
+    const int outcome = shade_chunk<ONE, FOLD, true>(C, g, a == ADV_QUERY ? s : -1, meta, ctl, ro, rd, st, n_inline);
+    if (a == ADV_QUERY && outcome != V_DONE) S.acc[s] = f4(L.acc);
     wave_add_u64(n_inline, &C.q.stats[(1 + g) * kStatLine + ST_ANY]);
 }
 
