@@ -276,8 +276,10 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScen
 
 // The deferred glass vertices: one lane per task (grid-stride over 64-task
 // chunks), glass_task's subtree and sample value into the sample's slot.
+// 4 waves per SIMD (128 VGPRs): the soft-light Cornell + mesh frame 3 %
+// faster than at 3 (156 VGPRs); at 5 (96 VGPRs) 4 % slower.
 template <bool COUNT>
-__global__ __launch_bounds__(kBlock, 3) void k_glass(DevScene S, RSettings st, const float4* __restrict__ tasks,
+__global__ __launch_bounds__(kBlock, 4) void k_glass(DevScene S, RSettings st, const float4* __restrict__ tasks,
                                                      const uint32_t* count, uint32_t cap, float* samp, int64_t n_px,
                                                      unsigned long long* ctr) {
     __shared__ unsigned long long lds[kStackK * kBlock];

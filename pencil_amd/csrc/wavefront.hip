@@ -966,7 +966,7 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
 // The finish launch's deferred glass vertices: one lane per task (grid-stride
 // over 64-task chunks), glass_task's subtree in the task's item's scene, the
 // sample's value into its slot.
-__global__ __launch_bounds__(kWfBlock, 3) void k_finish_glass(Ctx C, const float4* __restrict__ tasks,
+__global__ __launch_bounds__(kWfBlock, 4) void k_finish_glass(Ctx C, const float4* __restrict__ tasks,
                                                              const uint32_t* count, uint32_t cap, float* samp,
                                                              int64_t n_px) {
     __shared__ unsigned long long lds[kStackK * kWfBlock];
