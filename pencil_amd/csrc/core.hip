@@ -804,14 +804,16 @@ int pcl_trace_rays_counts(pcl_ctx* ctx, int32_t scene, const pcl_ray* d_rays, in
     return PCL_OK;
 } PCL_CATCH(ctx)
 
+constexpr int kMaxWalkDraws = 1 << 15;   // per shaded node, for deferral with the walk
+
 // Generator draws of one shaded Whitted node (whitted_local): a unit_disc
-// (2 draws) per sample of every point light with a radius (the host copy of
-// render.hpp's whitted_draws).
+// (2 draws) per sample of every point light with a radius.  Passed to the
+// deferred programs (GlassSink::draws, ItemRec::draws).
 static int whitted_draws_per_node(const SceneSlot& sl, const RSettings& st) {
-    int d = 0;
+    int64_t d = 0;
     for (const LightRec& L : sl.lights)
-        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) d += 2 * st.soft_samples;
-    return d;
+        if (L.kind == PCL_LIGHT_POINT && L.radius > 0.0f) d += 2 * (int64_t)st.soft_samples;
+    return (int)std::min<int64_t>(d, INT32_MAX);
 }
 
 // Deferred glass vertices (render_pixel_defer): path tracing, a scene with a
@@ -821,6 +823,8 @@ static int whitted_draws_per_node(const SceneSlot& sl, const RSettings& st) {
 // of the device's free memory.  Returns false (render in place) otherwise.
 static bool defer_buffers(pcl_ctx* ctx, const SceneSlot& sl, const RSettings& st) {
     if (!st.pathtrace || !sl.has_glass || ctx->defer_glass == 0) return false;
+    // a subtree's draws (draws per node x at most 2^16 nodes) stay below 2^31
+    if (whitted_draws_per_node(sl, st) > kMaxWalkDraws) return false;
     const int64_t n_px = (int64_t)st.width * st.height, nn = (int64_t)st.samples * st.samples;
     if (n_px * nn >= ((int64_t)1 << 32)) return false;   // the task counter is 32-bit
     const size_t samp = (size_t)(n_px * nn * 3);

@@ -527,7 +527,8 @@ PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& 
         Ray rr = push_glass(F, nf, P.cur, rec, albedo, ior, P.depth, st.shadow_bias);
         if (sink && sink->append(F[0], rr, P.T, P.R, emission, rng.s)) {
             if constexpr (WALK) {
-                if (sink->draws) rng.skip((uint32_t)(sink->draws * whitted_shaded_nodes(S, st, rr, P.depth - 1, F, nf, stk)));
+                if (sink->draws)
+                    rng.skip((uint32_t)sink->draws * (uint32_t)whitted_shaded_nodes(S, st, rr, P.depth - 1, F, nf, stk));
             }
             P.deferred = true;
             return false;

@@ -1242,11 +1242,12 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     bool any_path = false, any_whitted = false;
     for (const auto& it : its) (it.st.pathtrace ? any_path : any_whitted) = true;
     // finish kernel over records [from .. to) on stream st
-    bool any_glass = false, any_draws = false;
+    bool any_glass = false, any_draws = false, draws_ok = true;
     int64_t nn_max = 1;
     for (const auto& it : its) {
         any_glass |= it.glass != 0;
         any_draws |= it.draws > 0;
+        draws_ok &= it.draws <= (1 << 15);   // a subtree's draws stay below 2^31
         nn_max = std::max<int64_t>(nn_max, (int64_t)it.st.samples * it.st.samples);
     }
     // Deferred glass vertices in the finish launch (as the megakernel's): for
@@ -1257,7 +1258,8 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
         const dim3 grid((to - from + kWfBlock - 1) / kWfBlock);
         FinishDefer fd{};
         const int64_t n_px = (int64_t)(to - from);
-        deferred = any_path && !any_whitted && any_glass && defer_glass != 0 && defer_buffers(n_px, nn_max);
+        deferred = any_path && !any_whitted && any_glass && draws_ok && defer_glass != 0 &&
+                   defer_buffers(n_px, nn_max);
         if (deferred) {
             hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, d_gcount, 0u);
             fd.sink.tasks = d_gtasks;

@@ -267,3 +267,20 @@ def test_runtime_toggle_equals_the_environment():
         core.free_scene(sid)
     finally:
         core.close()
+
+
+def test_huge_light_sample_counts_render_in_place(cores):
+    """Above 2^15 generator draws per shaded node (here 2 soft lights x 20,000
+    jittered samples) a subtree's draws could pass 2^31, so glass vertices are
+    evaluated in place; the image is the in-place program's."""
+    c_off, c_on, _ = cores
+    hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=16, height=12, samples=1)
+    st = hs.settings()
+    st.pathtrace, st.pathtrace_bounces, st.soft_shadow_samples = 1, 4, 20000
+    d = hs.desc()
+    assert d.n_lights >= 2
+    d.lights[0].radius = d.lights[1].radius = 0.3
+    ref, sr = _render(c_off, hs, st)
+    img, s = _render(c_on, hs, st)
+    _same(img, s, ref, sr)
+    assert s.deferred_glass == 0
