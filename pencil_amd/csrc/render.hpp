@@ -242,9 +242,10 @@ struct GlassSink {
     int32_t draws;       // generator draws per shaded Whitted node
     int32_t pixel, k;    // the sample being traced
     int32_t item = 0;    // batch item of the pixel (the batched tracer's finish launch)
-    // Wave-aggregated append (every lane of the branch calls it); false when
-    // the task array is full and the caller evaluates the vertex itself.
-    PCL_D bool append(const Frame& f, const Ray& rr, const f3& T, const f3& R, const f3& e, uint32_t rng) {
+    // Wave-aggregated append (every lane of the branch calls it); the task's
+    // index, or -1 when the task array is full and the caller evaluates the
+    // vertex itself.
+    PCL_D int append(const Frame& f, const Ray& rr, const f3& T, const f3& R, const f3& e, uint32_t rng) {
         const unsigned long long m = __ballot(1);
         const int lane = threadIdx.x & 63;
         const int leader = __ffsll((long long)m) - 1;
@@ -252,7 +253,7 @@ struct GlassSink {
         if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
         base = __shfl(base, leader, 64);
         const uint32_t idx = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        if (idx >= cap) return false;
+        if (idx >= cap) return -1;
         float4* q = tasks + (size_t)idx * 8;
         q[0] = make_float4(rr.o.x, rr.o.y, rr.o.z, __int_as_float(f.depth));
         q[1] = make_float4(rr.d.x, rr.d.y, rr.d.z, __int_as_float(f.type));
@@ -262,7 +263,7 @@ struct GlassSink {
         q[5] = make_float4(T.x, T.y, T.z, __int_as_float(k));
         q[6] = make_float4(R.x, R.y, R.z, __uint_as_float(rng));
         q[7] = make_float4(e.x, e.y, e.z, __int_as_float(item));
-        return true;
+        return (int)idx;
     }
 };
 
@@ -415,56 +416,164 @@ struct PathState {
     f3 T, R;
     int depth;
     bool deferred = false;   // the walk ended at a deferred glass vertex
+    int walk_task = -1;      // that vertex's task, when its subtree is still to be walked
 };
 
 struct NoSink {
     int32_t draws = 0;
-    PCL_D bool append(const Frame&, const Ray&, const f3&, const f3&, const f3&, uint32_t) { return false; }
+    PCL_D int append(const Frame&, const Ray&, const f3&, const f3&, const f3&, uint32_t) { return -1; }
 };
 
-// The number of nodes of the Whitted recursion whitted(S, st, r, depth, ., F,
-// nf) evaluates (renderer.hpp:403-822, trace_dielectric :1582-1627) whose
-// closest hit succeeds at depth > 0 -- the nodes that run the light loop and
-// draw -- found by tracing the same rays in the same order without lights,
-// shading or colours.  Uses and changes the frames from F[nf] down.
-PCL_D int whitted_shaded_nodes(const DevScene& S, const RSettings& st, Ray r, int depth, Frame* F, int nf,
-                               Stack& stk) {
+// A node of a deferred subtree's walk: ray, depth of the trace() call, the
+// lane whose glass vertex the subtree belongs to, and (lanes of different
+// scenes in one wave: the batched tracer's finish launch) that lane's scene
+// and shadow bias.
+struct WalkNode {
+    f3 o, d;
+    int depth, owner;
+};
+template <bool MULTI>
+struct WalkEntry : WalkNode {};
+template <>
+struct WalkEntry<true> : WalkNode {
+    const DevScene* scene;
+    float bias;
+};
+
+// One walked node: the closest hit of trace(r, depth) and the trace() calls
+// it makes -- a metal child, or a glass node's reflection and (unless total
+// internal reflection) refraction, as whitted() spawns them -- when they
+// query (depth - 1 > 0).  Returns the number of children (0..2).
+template <bool MULTI>
+PCL_D int walk_node(const DevScene& S, float bias, const WalkEntry<MULTI>& e, Stack& stk, bool& hit,
+                    WalkEntry<MULTI>& c1, WalkEntry<MULTI>& c2) {
     LaneCount<false> none;   // the rays are counted where the subtree is evaluated
-    int n = 0;
-    while (true) {
-        bool child = false;
-        if (depth > 0) {
-            HitRec rec;
-            if (scene_hit(S, r, 1e-4f, 1e9f, rec, stk, none)) {
-                ++n;
-                int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
-                load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
-                if (kind == PCL_MAT_METAL && refl > 0.0f) {
-                    Frame& f = F[nf++];
-                    f.type = FR_METAL;
-                    f.depth = depth;
-                    Ray rr;
-                    rr.o = rec.p + rec.normal * st.shadow_bias;
-                    rr.d = r.d.normalized().reflect(rec.normal).normalized();
-                    r = rr;
-                    depth -= 1;
-                    child = true;
-                } else if (kind == PCL_MAT_DIELECTRIC) {
-                    r = push_glass(F, nf, r, rec, albedo, ior, depth, st.shadow_bias);
-                    depth -= 1;
-                    child = true;
-                }
+    HitRec rec;
+    Ray r;
+    r.o = e.o; r.d = e.d;
+    hit = scene_hit(S, r, 1e-4f, 1e9f, rec, stk, none);
+    if (!hit || e.depth - 1 <= 0) return 0;
+    int kind; f3 albedo, emission, spec_color; float refl, ior, spec, shin;
+    load_mat(S.mats, rec.mat, kind, albedo, emission, refl, ior, spec, shin, spec_color);
+    if constexpr (MULTI) {
+        c1.scene = c2.scene = e.scene;
+        c1.bias = c2.bias = e.bias;
+    }
+    if (kind == PCL_MAT_METAL && refl > 0.0f) {
+        c1.o = rec.p + rec.normal * bias;
+        c1.d = r.d.normalized().reflect(rec.normal).normalized();
+        c1.depth = e.depth - 1; c1.owner = e.owner;
+        return 1;
+    }
+    if (kind != PCL_MAT_DIELECTRIC) return 0;
+    Frame f[1];
+    int nf = 0;
+    const Ray rr = push_glass(f, nf, r, rec, albedo, ior, e.depth, bias);
+    c1.o = rr.o; c1.d = rr.d; c1.depth = e.depth - 1; c1.owner = e.owner;
+    if (f[0].type != FR_GLASS_REFL) return 1;
+    c2.o = f[0].ro; c2.d = f[0].rd; c2.depth = e.depth - 1; c2.owner = e.owner;
+    return 2;
+}
+
+// Index of the n-th set bit of m (n < popcount(m)).
+PCL_D int nth_bit(unsigned long long m, uint32_t n) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint32_t c = (uint32_t)__popcll(m & ((1ull << w) - 1ull));
+        if (n >= c) { n -= c; m >>= w; pos += w; }
+    }
+    return pos;
+}
+
+template <bool MULTI>
+PCL_D WalkEntry<MULTI> shfl_entry(const WalkEntry<MULTI>& e, int src) {
+    WalkEntry<MULTI> r;
+    r.o = f3(__shfl(e.o.x, src, 64), __shfl(e.o.y, src, 64), __shfl(e.o.z, src, 64));
+    r.d = f3(__shfl(e.d.x, src, 64), __shfl(e.d.y, src, 64), __shfl(e.d.z, src, 64));
+    r.depth = __shfl(e.depth, src, 64);
+    r.owner = __shfl(e.owner, src, 64);
+    if constexpr (MULTI) {
+        const unsigned long long a = (unsigned long long)(uintptr_t)e.scene;
+        const uint32_t lo = __shfl((uint32_t)a, src, 64), hi = __shfl((uint32_t)(a >> 32), src, 64);
+        r.scene = reinterpret_cast<const DevScene*>((uintptr_t)(((unsigned long long)hi << 32) | lo));
+        r.bias = __shfl(e.bias, src, 64);
+    }
+    return r;
+}
+
+// The walks of the wave's deferred glass vertices of this sample, together
+// (every active lane of the wave calls it at the sample's end, where they
+// reconverge).  The number of nodes a subtree shades does not depend on the
+// order its nodes are visited in, so each lane walks depth-first from its
+// current node, keeps one sibling as a spare (further ones on a private
+// stack: at most one per depth level), and a lane out of work takes another
+// lane's spare; the shaded nodes are counted per owning lane.  An owner then
+// skips its generator past draws x count values, as the subtree's
+// evaluation will draw them.  MULTI: the wave's lanes may render different
+// scenes; a node is traced in its owner's scene.
+template <bool MULTI>
+PCL_D void walk_wave(const DevScene& S, const RSettings& st, const GlassSink& sink, int task, Rng& rng, Stack& stk) {
+    const unsigned long long pend = __ballot(task >= 0);
+    if (!pend) return;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    bool hc = false, hs = false;   // a current node, a spare node
+    WalkEntry<MULTI> cur, spare;
+    cur.o = cur.d = spare.o = spare.d = f3(0, 0, 0);
+    cur.depth = spare.depth = 0; cur.owner = spare.owner = lane;
+    if constexpr (MULTI) {
+        cur.scene = spare.scene = &S;
+        cur.bias = spare.bias = st.shadow_bias;
+    }
+    WalkEntry<MULTI> priv[kMaxFrames];
+    int sp = 0;
+    uint32_t count = 0;
+    if (task >= 0) {   // the glass frame's two calls: reflection, then refraction
+        const float4* t = sink.tasks + (size_t)task * 8;
+        const float4 a = t[0], b = t[1], e = t[3], g4 = t[4];
+        const int cd = __float_as_int(a.w) - 1;
+        if (cd > 0) {
+            cur.o = f3(a.x, a.y, a.z); cur.d = f3(b.x, b.y, b.z); cur.depth = cd; hc = true;
+            if (__float_as_int(b.w) == FR_GLASS_REFL) {
+                spare.o = f3(e.x, e.y, e.z); spare.d = f3(g4.x, g4.y, g4.z); spare.depth = cd; hs = true;
             }
         }
-        if (child) continue;
-        // the return path: a reflected glass child is followed by its refraction sibling
-        while (nf > 0 && F[nf - 1].type != FR_GLASS_REFL) --nf;
-        if (nf == 0) return n;
-        Frame& f = F[nf - 1];
-        f.type = FR_GLASS_REFR;
-        r.o = f.ro; r.d = f.rd;
-        depth = f.depth - 1;
     }
+    while (true) {
+        if (!hc && sp > 0) { cur = priv[--sp]; hc = true; }
+        if (!hc && hs) { cur = spare; hc = true; hs = false; }
+        if (!__ballot(hc || hs)) break;   // private stacks are empty once hc is false
+        const unsigned long long idle = __ballot(!hc), donors = __ballot(hs);
+        if (idle && donors) {
+            const uint32_t m = min((uint32_t)__popcll(idle), (uint32_t)__popcll(donors));
+            const uint32_t ri = (uint32_t)__popcll(idle & lt), rd = (uint32_t)__popcll(donors & lt);
+            const int src = (!hc && ri < m) ? nth_bit(donors, ri) : lane;
+            const WalkEntry<MULTI> got = shfl_entry<MULTI>(spare, src);
+            if (!hc && ri < m) { cur = got; hc = true; }
+            if (hs && rd < m) hs = false;
+        }
+        bool hit = false;
+        const int owner = cur.owner;
+        if (hc) {
+            WalkEntry<MULTI> c1, c2;
+            int nc;
+            if constexpr (MULTI) nc = walk_node<true>(*cur.scene, cur.bias, cur, stk, hit, c1, c2);
+            else nc = walk_node<false>(S, st.shadow_bias, cur, stk, hit, c1, c2);
+            hc = nc >= 1;
+            if (hc) cur = c1;
+            if (nc == 2) {
+                if (!hs) { spare = c2; hs = true; }
+                else priv[sp++] = c2;
+            }
+        }
+        for (unsigned long long po = pend; po; po &= po - 1) {
+            const int o = __ffsll((long long)po) - 1;
+            const uint32_t c = (uint32_t)__popcll(__ballot(hit && owner == o));
+            if (lane == o) count += c;
+        }
+    }
+    if (task >= 0) rng.skip((uint32_t)sink.draws * count);
 }
 
 // One iteration of trace_path's bounce loop (its caller checks depth > 0):
@@ -525,11 +634,9 @@ PCL_D bool path_step(const DevScene& S, const RSettings& st, PathState& P, Rng& 
     } else if (kind == PCL_MAT_DIELECTRIC) {
         int nf = 0;
         Ray rr = push_glass(F, nf, P.cur, rec, albedo, ior, P.depth, st.shadow_bias);
-        if (sink && sink->append(F[0], rr, P.T, P.R, emission, rng.s)) {
-            if constexpr (WALK) {
-                if (sink->draws)
-                    rng.skip((uint32_t)sink->draws * (uint32_t)whitted_shaded_nodes(S, st, rr, P.depth - 1, F, nf, stk));
-            }
+        const int ti = sink ? sink->append(F[0], rr, P.T, P.R, emission, rng.s) : -1;
+        if (ti >= 0) {
+            if constexpr (WALK) P.walk_task = ti;   // walked with the wave when the sample ends
             P.deferred = true;
             return false;
         }
@@ -604,7 +711,7 @@ PCL_D f3 render_pixel(const DevScene& S, const CamRec& C, const RSettings& st, i
 // program without the walk is compiled for scenes whose Whitted draws nothing
 // (the walk's code alone, never run, costs the megakernel 8 % on the hard-light
 // Cornell + mesh frame).
-template <bool WALK, class LC>
+template <bool WALK, bool MULTI, class LC>
 PCL_D bool render_samples_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, int k0,
                                 Rng& rng, f3 acc, Frame* F, Stack& stk, LC& c, GlassSink& sink, float* samp,
                                 size_t stride, f3& px, int& first) {
@@ -624,6 +731,7 @@ PCL_D bool render_samples_defer(const DevScene& S, const CamRec& C, const RSetti
         while (P.depth > 0) {
             if (!path_step<LC, GlassSink, WALK>(S, st, P, rng, F, stk, c, &sink)) break;
         }
+        if constexpr (WALK) walk_wave<MULTI>(S, st, sink, P.walk_task, rng, stk);
         if (P.deferred) {
             if (first == n * n) first = k;
         } else if (first == n * n) {
@@ -642,8 +750,8 @@ template <bool WALK, class LC>
 PCL_D bool render_pixel_defer(const DevScene& S, const CamRec& C, const RSettings& st, int i, int j, Frame* F,
                               Stack& stk, LC& c, GlassSink& sink, float* samp, size_t stride, f3& px, int& first) {
     Rng rng((uint32_t)(i * 97861u + j * 1973u + 1u));
-    return render_samples_defer<WALK>(S, C, st, i, j, 0, rng, f3(0, 0, 0), F, stk, c, sink, samp, stride, px,
-                                      first);
+    return render_samples_defer<WALK, false>(S, C, st, i, j, 0, rng, f3(0, 0, 0), F, stk, c, sink, samp, stride,
+                                             px, first);
 }
 
 // One deferred glass vertex (a GlassSink task, 8 float4 at q): the Whitted

@@ -942,8 +942,9 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
             sink.pixel = (int32_t)p;
             sink.item = m.x;
             int first;
-            done = render_samples_defer<DEFER == 2>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]), F,
-                                                    stk, c, sink, fd.samp + p * 3, (size_t)fd.n_px * 3, px, first);
+            done = render_samples_defer<DEFER == 2, true>(D, it.cam, it.st, m.y % W, m.y / W, m.z, rng, xyz(accs[e]),
+                                                          F, stk, c, sink, fd.samp + p * 3, (size_t)fd.n_px * 3, px,
+                                                          first);
             fd.first[p] = first;
             if (!done) {
                 float* q = fd.prefix + p * 3;
