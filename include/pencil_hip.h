@@ -268,7 +268,7 @@ typedef struct pcl_wf_config {
     int32_t drain_below;      /* once every job pixel is taken and fewer slots than this are
                                  active, each slot stops at its next sample boundary and one
                                  per-pixel launch finishes the remaining samples; -1 = never,
-                                 0 = the traversal grids' lanes (CUs x blocks_per_cu x 128) */
+                                 0 = four times the traversal grids' lanes (CUs x blocks_per_cu x 512) */
     int32_t inline_shadows;   /* a vertex with at most this many shadow rays traces them in the
                                  closest-hit kernel instead of the shadow queue; -1 = never */
     int32_t streams;          /* engines the job's 64-pixel tiles are dealt to (interleaved), each

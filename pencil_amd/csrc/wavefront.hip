@@ -51,6 +51,7 @@ namespace wf {
 
 constexpr int kWave = 64;
 constexpr int kWfBlock = 128;
+constexpr int kDrainLanes = 4;   // automatic drain: this many times the traversal grids' lanes
 
 enum : int { M_PATH = 0, M_WHITTED = 1 };
 // vertex outcome decided by shade, consumed by resolve
@@ -1181,9 +1182,13 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
     if (cfg.blocks_per_cu <= 0) cfg.blocks_per_cu = 8;
     if (cfg.stack_k != 8 && cfg.stack_k != 16 && cfg.stack_k != 32) cfg.stack_k = 16;
     if (cfg.iters_per_graph <= 0) cfg.iters_per_graph = 4;
-    // auto: drain once the active slots would no longer fill the persistent
-    // traversal grids (blocks_per_cu blocks of kWfBlock lanes per CU)
-    if (cfg.drain_below == 0) cfg.drain_below = cus * cfg.blocks_per_cu * kWfBlock;
+    // auto: drain once fewer slots are active than four times the persistent
+    // traversal grids' lanes (blocks_per_cu blocks of kWfBlock lanes per CU):
+    // with its glass vertices deferred the finish launch runs the tail
+    // better than iterations with that few slots (Cornell + mesh, 256 spp:
+    // 2.39 s, against 2.45 s at once the grids' lanes, 2.48 s at six times;
+    // the 64-scene job the same at one to four times)
+    if (cfg.drain_below == 0) cfg.drain_below = kDrainLanes * cus * cfg.blocks_per_cu * kWfBlock;
     if (cfg.inline_shadows == 0) cfg.inline_shadows = 1;
     int rc = ensure(cfg.slots, (int)items.size(), err);
     if (rc) return rc;
@@ -1421,7 +1426,7 @@ int Tracer::run(int device, hipStream_t stream, const std::vector<ItemRec>& item
     if (cfg.slots <= 0) cfg.slots = (int)std::min<int64_t>(total, 2 << 20);
     cfg.slots = (int)std::max<int64_t>(1, std::min<int64_t>(cfg.slots, total));
     if (cfg.blocks_per_cu <= 0) cfg.blocks_per_cu = 8;
-    if (cfg.drain_below == 0) cfg.drain_below = cus * cfg.blocks_per_cu * kWfBlock;
+    if (cfg.drain_below == 0) cfg.drain_below = kDrainLanes * cus * cfg.blocks_per_cu * kWfBlock;
     if (cfg.streams <= 0) {
         // Concurrent parts pay where iterations repeat: a one-sample job runs
         // a handful of iterations and is faster as one engine (headline frame
