@@ -514,7 +514,8 @@ PCL_D WalkEntry<MULTI> shfl_entry(const WalkEntry<MULTI>& e, int src) {
 // scenes; a node is traced in its owner's scene.
 template <bool MULTI>
 PCL_D void walk_wave(const DevScene& S, const RSettings& st, const GlassSink& sink, int task, Rng& rng, Stack& stk) {
-    const unsigned long long pend = __ballot(task >= 0);
+    const bool mine = task >= 0 && sink.draws > 0;   // a subtree that draws is pending
+    const unsigned long long pend = __ballot(mine);
     if (!pend) return;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
@@ -529,7 +530,7 @@ PCL_D void walk_wave(const DevScene& S, const RSettings& st, const GlassSink& si
     WalkEntry<MULTI> priv[kMaxFrames];
     int sp = 0;
     uint32_t count = 0;
-    if (task >= 0) {   // the glass frame's two calls: reflection, then refraction
+    if (mine) {   // the glass frame's two calls: reflection, then refraction
         const float4* t = sink.tasks + (size_t)task * 8;
         const float4 a = t[0], b = t[1], e = t[3], g4 = t[4];
         const int cd = __float_as_int(a.w) - 1;
@@ -573,7 +574,7 @@ PCL_D void walk_wave(const DevScene& S, const RSettings& st, const GlassSink& si
             if (lane == o) count += c;
         }
     }
-    if (task >= 0) rng.skip((uint32_t)sink.draws * count);
+    if (mine) rng.skip((uint32_t)sink.draws * count);
 }
 
 // One iteration of trace_path's bounce loop (its caller checks depth > 0):

@@ -178,10 +178,11 @@ def _batch_case(core, items, cfg):
 
 
 def _batch_items(hard, whitted):
+    """hard: True, False, or "mixed" (the first and last scene hard, the middle one soft)."""
     items = []
-    for name, w, h, side, pt, b in (("cornell_c5", 96, 72, 3, 1, 8), ("glass", 64, 48, 2, 2, 6),
-                                    ("three_spheres", 40, 30, 4, 1, 8)):
-        items.append(_scene(name, w, h, side, pt, b, hard))
+    for n, (name, w, h, side, pt, b) in enumerate((("cornell_c5", 96, 72, 3, 1, 8), ("glass", 64, 48, 2, 2, 6),
+                                                   ("three_spheres", 40, 30, 4, 1, 8))):
+        items.append(_scene(name, w, h, side, pt, b, (n != 1) if hard == "mixed" else hard))
     if whitted:
         hs = pa.HostScene.from_pecl(golden("glass.pecl"), width=48, height=36, samples=2)
         items.append((hs, hs.settings()))
@@ -190,7 +191,7 @@ def _batch_items(hard, whitted):
 
 @pytest.mark.parametrize("drain", [900, 1 << 30])
 @pytest.mark.parametrize("streams", [1, 3])
-@pytest.mark.parametrize("hard", [True, False], ids=["hard", "soft"])
+@pytest.mark.parametrize("hard", [True, False, "mixed"], ids=["hard", "soft", "mixed"])
 def test_batched_finish_defers_glass_bit_for_bit(cores, hard, streams, drain):
     """The batched tracer's drain: its finish launch defers the glass vertices
     of path-traced jobs (three scenes of different sizes and sample counts,
