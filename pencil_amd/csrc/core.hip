@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "adjoint_launch.hpp"
+#include "glass_sort.hpp"
 #include "tonemap_launch.hpp"
 #include "host_build.hpp"
 #include "lbvh.hpp"
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(kBlock, RenderWaves<PT>::v) void k_render_p(DevScen
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock, 4) void k_glass(DevScene S, RSettings st, const float4* __restrict__ tasks,
                                                      const uint32_t* count, uint32_t cap, float* samp, int64_t n_px,
-                                                     unsigned long long* ctr) {
+                                                     unsigned long long* ctr, const uint32_t* __restrict__ perm) {
     __shared__ unsigned long long lds[kStackK * kBlock];
     const uint32_t n = min(*count, cap);
     LaneCount<COUNT> c;
@@ -292,8 +293,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_glass(DevScene S, RSettings st, c
     stk.set_stride(kBlock);
     const uint32_t wave = (blockIdx.x * kBlock + threadIdx.x) >> 6, waves = (gridDim.x * kBlock) >> 6;
     for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
-        const uint32_t t = base + (threadIdx.x & 63);
-        if (t >= n) continue;
+        const uint32_t i = base + (threadIdx.x & 63);
+        if (i >= n) continue;
+        const uint32_t t = perm ? perm[i] : i;
         const float4* q = tasks + (size_t)t * 8;
         const f3 v = glass_task(S, st, q, F, stk, c);
         const int64_t p = __float_as_int(q[4].w), k = __float_as_int(q[5].w);
@@ -365,6 +367,10 @@ struct pcl_ctx {
     // grown on demand: tasks, per-sample values, per-pixel prefix / first
     float4* d_gtasks = nullptr;
     size_t gtask_cap = 0;
+    uint32_t* d_gkeys = nullptr;          // per task: its subtree's shaded nodes (walked renders)
+    uint32_t* d_gperm = nullptr;          // tasks in the glass launch's order
+    size_t gkey_cap = 0;
+    uint32_t* d_gbins = nullptr;          // 256 counts + 256 offsets of the task sort
     uint32_t* d_gcount = nullptr;         // one 128-byte line
     float* d_samp = nullptr;
     size_t samp_cap = 0;                  // floats
@@ -519,6 +525,8 @@ void pcl_destroy(pcl_ctx* ctx) {
     if (ctx->d_thr) (void)hipFree(ctx->d_thr);
     if (ctx->d_rgb8) (void)hipFree(ctx->d_rgb8);
     if (ctx->d_heads) (void)hipFree(ctx->d_heads);
+    for (void* p : {(void*)ctx->d_gkeys, (void*)ctx->d_gperm, (void*)ctx->d_gbins})
+        if (p) (void)hipFree(p);
     for (void* p : {(void*)ctx->d_gtasks, (void*)ctx->d_gcount, (void*)ctx->d_samp, (void*)ctx->d_prefix,
                     (void*)ctx->d_first})
         if (p) (void)hipFree(p);
@@ -832,7 +840,7 @@ static bool defer_buffers(pcl_ctx* ctx, const SceneSlot& sl, const RSettings& st
                                               : (size_t)std::min<int64_t>(n_px * nn / 8 + 4096, (int64_t)1 << 27);
     size_t need = 0;
     if (samp > ctx->samp_cap) need += samp * sizeof(float);
-    if (tasks > ctx->gtask_cap) need += tasks * 8 * sizeof(float4);
+    if (tasks > ctx->gtask_cap) need += tasks * (8 * sizeof(float4) + 2 * sizeof(uint32_t));
     if ((size_t)n_px > ctx->px_cap) need += (size_t)n_px * 16;
     if (need) {
         size_t free_b = 0, total_b = 0;
@@ -855,7 +863,11 @@ static bool defer_buffers(pcl_ctx* ctx, const SceneSlot& sl, const RSettings& st
     size_t pxc2 = ctx->px_cap;
     if (!grow(&ctx->d_first, pxc2, (size_t)n_px, sizeof(int32_t))) return false;
     ctx->px_cap = std::min(pxc, pxc2);
-    return true;
+    size_t kc = ctx->gkey_cap, kc2 = ctx->gkey_cap, bc = ctx->d_gbins ? 512 : 0;
+    const bool ok = grow(&ctx->d_gkeys, kc, tasks, sizeof(uint32_t)) && grow(&ctx->d_gperm, kc2, tasks, sizeof(uint32_t)) &&
+                    grow(&ctx->d_gbins, bc, 512, sizeof(uint32_t));
+    ctx->gkey_cap = std::min(kc, kc2);
+    return ok;
 }
 
 static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings,
@@ -879,9 +891,11 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     if (defer) {
         db.sink.tasks = ctx->d_gtasks;
         db.sink.count = ctx->d_gcount;
-        db.sink.cap = (uint32_t)std::min<size_t>(ctx->gtask_cap, ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks
-                                                                                       : ctx->gtask_cap);
+        // keys and tasks share the capacity (either array may have grown alone before a failed call)
+        const size_t tcap = std::min(ctx->gtask_cap, ctx->gkey_cap);
+        db.sink.cap = (uint32_t)std::min<size_t>(tcap, ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks : tcap);
         db.sink.draws = whitted_draws_per_node(*sl, st);
+        db.sink.keys = db.sink.draws > 0 ? ctx->d_gkeys : nullptr;   // the walk's counts order the glass launch
         db.samp = ctx->d_samp;
         db.prefix = ctx->d_prefix;
         db.first = ctx->d_first;
@@ -926,11 +940,17 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     }
     HIP_TRY(ctx, hipGetLastError());
     if (defer) {
+        const uint32_t* perm = nullptr;
+        if (db.sink.keys) {   // tasks ordered by subtree size (glass_sort.hpp)
+            glass_sort(ctx->d_gkeys, ctx->d_gcount, db.sink.cap, ctx->d_gbins, ctx->d_gperm, ctx->stream);
+            HIP_TRY(ctx, hipGetLastError());
+            perm = ctx->d_gperm;
+        }
         int nb = 0;
         HIP_TRY(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, counts ? k_glass<true> : k_glass<false>, kBlock, 0));
         const int ggrid = ctx->cus * std::max(nb, 1);
         hipLaunchKernelGGL(counts ? k_glass<true> : k_glass<false>, dim3(ggrid), dim3(kBlock), 0, ctx->stream, sl->dev,
-                           st, ctx->d_gtasks, ctx->d_gcount, db.sink.cap, ctx->d_samp, db.n_px, ctx->d_ctr);
+                           st, ctx->d_gtasks, ctx->d_gcount, db.sink.cap, ctx->d_samp, db.n_px, ctx->d_ctr, perm);
         HIP_TRY(ctx, hipGetLastError());
         hipLaunchKernelGGL(k_sum, dim3((unsigned)((db.n_px + 255) / 256)), dim3(256), 0, ctx->stream, d_out_rgb,
                            ctx->d_samp, ctx->d_prefix, ctx->d_first, db.n_px, st.samples * st.samples);

@@ -242,6 +242,7 @@ struct GlassSink {
     int32_t draws;       // generator draws per shaded Whitted node
     int32_t pixel, k;    // the sample being traced
     int32_t item = 0;    // batch item of the pixel (the batched tracer's finish launch)
+    uint32_t* keys = nullptr;   // when set: per task, the walk's count of the subtree's shaded nodes
     // Wave-aggregated append (every lane of the branch calls it); the task's
     // index, or -1 when the task array is full and the caller evaluates the
     // vertex itself.
@@ -574,7 +575,10 @@ PCL_D void walk_wave(const DevScene& S, const RSettings& st, const GlassSink& si
             if (lane == o) count += c;
         }
     }
-    if (mine) rng.skip((uint32_t)sink.draws * count);
+    if (mine) {
+        rng.skip((uint32_t)sink.draws * count);
+        if (sink.keys) sink.keys[task] = count;
+    }
 }
 
 // One iteration of trace_path's bounce loop (its caller checks depth > 0):

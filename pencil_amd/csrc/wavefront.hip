@@ -43,6 +43,7 @@
 #include <thread>
 #include <vector>
 
+#include "glass_sort.hpp"
 #include "render.hpp"
 #include "wavefront.hpp"
 
@@ -970,7 +971,7 @@ __global__ __launch_bounds__(kWfBlock, PT == 1 ? 4 : 3) void k_finish(Ctx C, con
 // sample's value into its slot.
 __global__ __launch_bounds__(kWfBlock, 4) void k_finish_glass(Ctx C, const float4* __restrict__ tasks,
                                                              const uint32_t* count, uint32_t cap, float* samp,
-                                                             int64_t n_px) {
+                                                             int64_t n_px, const uint32_t* __restrict__ perm) {
     __shared__ unsigned long long lds[kStackK * kWfBlock];
     const uint32_t n = min(*count, cap);
     LaneCount<false> c;
@@ -981,8 +982,9 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_finish_glass(Ctx C, const float
     stk.set_stride(kWfBlock);
     const uint32_t wave = (blockIdx.x * kWfBlock + threadIdx.x) >> 6, waves = (gridDim.x * kWfBlock) >> 6;
     for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
-        const uint32_t t = base + (threadIdx.x & 63);
-        if (t >= n) continue;
+        const uint32_t i = base + (threadIdx.x & 63);
+        if (i >= n) continue;
+        const uint32_t t = perm ? perm[i] : i;   // glass_sort.hpp's order under soft lights
         const float4* q = tasks + (size_t)t * 8;
         const ItemRec& it = C.items[__float_as_int(q[7].w)];
         const f3 v = glass_task(C.scenes[it.scene], it.st, q, F, stk, c);
@@ -1061,10 +1063,12 @@ void Engine::release() {
     bufs.clear();
     cap_slots = 0;
     cap_items = 0;
-    for (void* p : {(void*)d_gtasks, (void*)d_gcount, (void*)d_samp, (void*)d_prefix, (void*)d_first})
+    for (void* p : {(void*)d_gtasks, (void*)d_gcount, (void*)d_samp, (void*)d_prefix, (void*)d_first,
+                    (void*)d_gkeys, (void*)d_gperm, (void*)d_gbins})
         if (p) (void)hipFree(p);
     d_gtasks = nullptr; d_gcount = nullptr; d_samp = nullptr; d_prefix = nullptr; d_first = nullptr;
-    gtask_cap = samp_cap = px_cap = 0;
+    d_gkeys = d_gperm = d_gbins = nullptr;
+    gtask_cap = samp_cap = px_cap = gkey_cap = 0;
 }
 
 // The finish launch's deferred-glass buffers: 12 B per listed pixel and
@@ -1081,7 +1085,7 @@ bool Engine::defer_buffers(int64_t n_px, int64_t nn) {
                                          : std::min<size_t>((size_t)(n_px * nn / 4 + 4096), (size_t)1 << 27);
     size_t need = 0;
     if (samp > samp_cap) need += samp * sizeof(float);
-    if (tasks > gtask_cap) need += tasks * 8 * sizeof(float4);
+    if (tasks > gtask_cap) need += tasks * (8 * sizeof(float4) + 2 * sizeof(uint32_t));
     if ((size_t)n_px > px_cap) need += (size_t)n_px * 16;
     if (need) {
         size_t free_b = 0, total_b = 0;
@@ -1104,7 +1108,11 @@ bool Engine::defer_buffers(int64_t n_px, int64_t nn) {
         return false;
     }
     px_cap = std::min(pxc, pxc2);
-    return true;
+    size_t kc = gkey_cap, kc2 = gkey_cap, bc = d_gbins ? 512 : 0;
+    const bool ok = grow(&d_gkeys, kc, tasks, sizeof(uint32_t)) && grow(&d_gperm, kc2, tasks, sizeof(uint32_t)) &&
+                    grow(&d_gbins, bc, 512, sizeof(uint32_t));
+    gkey_cap = std::min(kc, kc2);
+    return ok;
 }
 
 int Engine::alloc(size_t bytes, void** p, std::string& err) {
@@ -1270,18 +1278,21 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
             hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, st, d_gcount, 0u);
             fd.sink.tasks = d_gtasks;
             fd.sink.count = d_gcount;
-            fd.sink.cap = (uint32_t)std::min<size_t>(gtask_cap, defer_tasks > 0 ? (size_t)defer_tasks : gtask_cap);
+            const size_t tcap = std::min(gtask_cap, gkey_cap);   // either may have grown alone before a failure
+            fd.sink.cap = (uint32_t)std::min<size_t>(tcap, defer_tasks > 0 ? (size_t)defer_tasks : tcap);
+            fd.sink.keys = any_draws ? d_gkeys : nullptr;
             fd.samp = d_samp;
             fd.prefix = d_prefix;
             fd.first = d_first;
             fd.n_px = n_px;
             if (any_draws) hipLaunchKernelGGL((k_finish<1, 2>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
             else hipLaunchKernelGGL((k_finish<1, 1>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
+            if (any_draws) glass_sort(d_gkeys, d_gcount, fd.sink.cap, d_gbins, d_gperm, st);
             int nb = 0;
             hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_finish_glass, kWfBlock, 0);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_finish_glass, dim3(cus * std::max(nb, 1)), dim3(kWfBlock), 0, st, c, d_gtasks,
-                               d_gcount, fd.sink.cap, d_samp, n_px);
+                               d_gcount, fd.sink.cap, d_samp, n_px, any_draws ? d_gperm : nullptr);
             hipLaunchKernelGGL(k_finish_sum, dim3((unsigned)((n_px + 255) / 256)), dim3(256), 0, st, c, m, (int)from,
                                d_samp, d_prefix, d_first, n_px);
             if ((e = hipGetLastError()) != hipSuccess) return e;

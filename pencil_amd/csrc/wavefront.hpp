@@ -150,6 +150,10 @@ struct Engine {
     // kept until release().
     float4* d_gtasks = nullptr;
     uint32_t* d_gcount = nullptr;
+    uint32_t* d_gkeys = nullptr;    // per task: the walk's count (soft lights), and the launch order
+    uint32_t* d_gperm = nullptr;
+    uint32_t* d_gbins = nullptr;    // 512 words of the sort
+    size_t gkey_cap = 0;
     float* d_samp = nullptr;
     float* d_prefix = nullptr;
     int32_t* d_first = nullptr;
