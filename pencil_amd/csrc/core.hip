@@ -895,7 +895,7 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
         const size_t tcap = std::min(ctx->gtask_cap, ctx->gkey_cap);
         db.sink.cap = (uint32_t)std::min<size_t>(tcap, ctx->defer_tasks > 0 ? (size_t)ctx->defer_tasks : tcap);
         db.sink.draws = whitted_draws_per_node(*sl, st);
-        db.sink.keys = db.sink.draws > 0 ? ctx->d_gkeys : nullptr;   // the walk's counts order the glass launch
+        db.sink.keys = ctx->d_gkeys;   // the glass launch's order
         db.samp = ctx->d_samp;
         db.prefix = ctx->d_prefix;
         db.first = ctx->d_first;

@@ -1,9 +1,10 @@
 // glass_sort.hpp - the order in which a glass launch takes its tasks.
 //
 // Under soft lights each deferred glass vertex's subtree was walked and its
-// shaded nodes counted (render.hpp: walk_wave, GlassSink::keys).  The glass
+// shaded nodes counted (render.hpp: walk_wave, GlassSink::keys); under hard
+// lights, with no walk, the key is the subtree's depth limit.  The glass
 // launches (core.hip: k_glass, wavefront.hip: k_finish_glass) take the tasks
-// by that count, largest first, so that a wave's lanes hold subtrees of about
+// by that key, largest first, so that a wave's lanes hold subtrees of about
 // one size instead of waiting for the largest of 64 random ones.  The values
 // do not depend on the order: each task writes its own sample.  A counting
 // sort over task indices: histogram, offsets, and a scatter that reserves

@@ -242,7 +242,9 @@ struct GlassSink {
     int32_t draws;       // generator draws per shaded Whitted node
     int32_t pixel, k;    // the sample being traced
     int32_t item = 0;    // batch item of the pixel (the batched tracer's finish launch)
-    uint32_t* keys = nullptr;   // when set: per task, the walk's count of the subtree's shaded nodes
+    // when set: per task, the walk's count of the subtree's shaded nodes
+    // (draws > 0), else the subtree's depth limit: the glass launch's order
+    uint32_t* keys = nullptr;
     // Wave-aggregated append (every lane of the branch calls it); the task's
     // index, or -1 when the task array is full and the caller evaluates the
     // vertex itself.
@@ -264,6 +266,7 @@ struct GlassSink {
         q[5] = make_float4(T.x, T.y, T.z, __int_as_float(k));
         q[6] = make_float4(R.x, R.y, R.z, __uint_as_float(rng));
         q[7] = make_float4(e.x, e.y, e.z, __int_as_float(item));
+        if (keys && draws == 0) keys[idx] = (uint32_t)f.depth;   // no walk: the subtree's depth limit
         return (int)idx;
     }
 };

@@ -984,7 +984,7 @@ __global__ __launch_bounds__(kWfBlock, 4) void k_finish_glass(Ctx C, const float
     for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
         const uint32_t i = base + (threadIdx.x & 63);
         if (i >= n) continue;
-        const uint32_t t = perm ? perm[i] : i;   // glass_sort.hpp's order under soft lights
+        const uint32_t t = perm ? perm[i] : i;   // the order of glass_sort.hpp
         const float4* q = tasks + (size_t)t * 8;
         const ItemRec& it = C.items[__float_as_int(q[7].w)];
         const f3 v = glass_task(C.scenes[it.scene], it.st, q, F, stk, c);
@@ -1280,19 +1280,19 @@ int Engine::run(hipStream_t stream, const std::vector<ItemRec>& items, const std
             fd.sink.count = d_gcount;
             const size_t tcap = std::min(gtask_cap, gkey_cap);   // either may have grown alone before a failure
             fd.sink.cap = (uint32_t)std::min<size_t>(tcap, defer_tasks > 0 ? (size_t)defer_tasks : tcap);
-            fd.sink.keys = any_draws ? d_gkeys : nullptr;
+            fd.sink.keys = d_gkeys;
             fd.samp = d_samp;
             fd.prefix = d_prefix;
             fd.first = d_first;
             fd.n_px = n_px;
             if (any_draws) hipLaunchKernelGGL((k_finish<1, 2>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
             else hipLaunchKernelGGL((k_finish<1, 1>), grid, dim3(kWfBlock), 0, st, c, m, a, (int)from, (int)to, fd);
-            if (any_draws) glass_sort(d_gkeys, d_gcount, fd.sink.cap, d_gbins, d_gperm, st);
+            glass_sort(d_gkeys, d_gcount, fd.sink.cap, d_gbins, d_gperm, st);
             int nb = 0;
             hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_finish_glass, kWfBlock, 0);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(k_finish_glass, dim3(cus * std::max(nb, 1)), dim3(kWfBlock), 0, st, c, d_gtasks,
-                               d_gcount, fd.sink.cap, d_samp, n_px, any_draws ? d_gperm : nullptr);
+                               d_gcount, fd.sink.cap, d_samp, n_px, d_gperm);
             hipLaunchKernelGGL(k_finish_sum, dim3((unsigned)((n_px + 255) / 256)), dim3(256), 0, st, c, m, (int)from,
                                d_samp, d_prefix, d_first, n_px);
             if ((e = hipGetLastError()) != hipSuccess) return e;
