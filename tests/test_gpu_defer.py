@@ -134,15 +134,20 @@ def test_mixed_lights_and_soft_sample_counts(cores, soft_samples):
         assert s.deferred_glass > 0
 
 
-def test_deferred_glass_persistent_grid_and_many_tasks():
+@pytest.mark.parametrize("hard", [False, True], ids=["soft", "hard"])
+def test_deferred_glass_persistent_grid_and_many_tasks(hard):
     """The persistent grid (forced) with deferral, on enough samples that the
-    glass launch strides over its grid more than once (> 100 k deferred
-    vertices): the in-place persistent render's image and counts."""
+    glass launch strides over its grid more than once and its ordering sort
+    runs over many blocks (> 100 k deferred vertices; keys: the walk's counts
+    under the soft light, the depth limits under the hard one): the in-place
+    persistent render's image and counts."""
     from pencil_amd import workloads as wl
     c_off = _core(PCL_DEFER_GLASS=0, PCL_RENDER_KERNEL=1)
     c_on = _core(PCL_DEFER_GLASS=1, PCL_RENDER_KERNEL=1)
     try:
         hs, st = wl.cornell_pt(pa, width=640, height=360, samples=4)
+        if hard:
+            _hard(hs)
         ref, sr = _render(c_off, hs, st)
         img, s = _render(c_on, hs, st)
     finally:
