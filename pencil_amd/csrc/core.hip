@@ -197,9 +197,9 @@ __global__ void k_zero2_u64(unsigned long long* a, int na, unsigned long long* b
     if (i < nb) b[i] = 0ull;
 }
 
-// ---- persistent render: waves pull 64-pixel chunks ------------------------
+// ---- persistent render: waves pull 8x8-pixel chunks -----------------------
 //
-// Chunk c is the 64-pixel row segment c of the image (row-major segments).
+// Chunk c is the 8x8-pixel tile c of the image (row-major tiles).
 // Chunks are dealt to kRStripes stripes round-robin (stripe g holds chunks g,
 // g + 64, ...), each stripe with its own head counter on its own 128-byte
 // line: device-scope atomics on one line serialize, so one counter for the
