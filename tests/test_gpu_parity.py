@@ -17,6 +17,7 @@ Bars (the parity contract, with the figures measured for them):
     (oracle_lib.trig_mode(1)) the core is bit-exact.
 """
 import json
+import os
 
 import numpy as np
 import pytest
@@ -417,3 +418,58 @@ def test_primitive_tests_match_reference(core, name):
         assert got == ref
     else:   # within 1e-3 of the reference's per kind (measured with the oracle in this rounding: <= 3.3e-5)
         assert all(abs(g - r) <= 1e-3 * max(r, 1) for g, r in zip(got, ref)), (got, ref)
+
+
+def _core_env(**env):
+    """A context created with the given environment settings."""
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return pa.Core(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.fixture(scope="module")
+def grid_cores():
+    """One context per render grid: one block per 128 pixels of a row, and
+    the persistent grid of waves pulling 8x8-pixel tiles."""
+    cores = [_core_env(PCL_RENDER_KERNEL=0), _core_env(PCL_RENDER_KERNEL=1)]
+    yield cores
+    for c in cores:
+        c.close()
+
+
+@pytest.mark.parametrize("size", [(1, 1), (7, 9), (65, 47), (100, 3), (9, 130)], ids=lambda s: f"{s[0]}x{s[1]}")
+@pytest.mark.parametrize("mode", ["whitted", "shallow", "path"])
+@pytest.mark.parametrize("name", ["cornell_box", "glass"])
+def test_persistent_grid_tiles_on_ragged_images(grid_cores, name, mode, size):
+    """The persistent grid deals the image in 8x8-pixel tiles and skips the
+    lanes of a tile that fall past the image's right or bottom edge: on
+    images whose sides are not multiples of 8 (and on a 1x1 image) it renders
+    what the block grid renders, bit for bit, with the same counts --
+    Whitted (depth as parsed and depth 1) and path tracing, whose glass
+    vertices the megakernel defers."""
+    w, h = size
+    hs = load(name, width=w, height=h, samples=2)
+    s = hs.settings()
+    if mode == "shallow":
+        s.max_depth = 1
+    elif mode == "path":
+        s.pathtrace, s.pathtrace_bounces = 1, 4
+    out = []
+    for c in grid_cores:
+        sid = c.upload(hs.desc())
+        try:
+            out.append(c.render(sid, hs.camera(), s))
+        finally:
+            c.free_scene(sid)
+    (a, sa), (b, sb) = out
+    assert a.shape == (h, w, 3)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (sa.rays_traced, sa.closest_queries, sa.anyhit_queries) == \
+        (sb.rays_traced, sb.closest_queries, sb.anyhit_queries)
