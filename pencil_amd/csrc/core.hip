@@ -344,6 +344,10 @@ struct pcl_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* d_ctr = nullptr;
+    // pinned host copy of a render's counters (kCtrSlots words) and its
+    // deferred-task count: a copy into pageable memory goes through a staging
+    // buffer on every frame
+    unsigned long long* h_ctr = nullptr;
     std::string err;
     std::vector<SceneSlot> scenes;
     pcl::wf::Tracer* wf = nullptr;
@@ -520,6 +524,7 @@ void pcl_destroy(pcl_ctx* ctx) {
     for (auto& s : ctx->scenes) if (s.used) release(s);
     delete ctx->wf;
     if (ctx->d_ctr) (void)hipFree(ctx->d_ctr);
+    if (ctx->h_ctr) (void)hipHostFree(ctx->h_ctr);
     if (ctx->d_adj_partial) (void)hipFree(ctx->d_adj_partial);
     if (ctx->d_adj_out) (void)hipFree(ctx->d_adj_out);
     if (ctx->d_img) (void)hipFree(ctx->d_img);
@@ -884,7 +889,9 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
     const CamRec C = make_camera(*cam);
     const RSettings st = to_dev(settings);
     const int tiles = ((st.width + 127) / 128) * st.height;   // 128-pixel row segments
-    unsigned long long h[kCtrSlots] = {0};
+    if (!ctx->h_ctr) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->h_ctr, (kCtrSlots + 1) * sizeof(unsigned long long),
+                                                hipHostMallocDefault));
+    unsigned long long* h = ctx->h_ctr;
     const int n_chunks = ((st.width + 7) / 8) * ((st.height + 7) / 8);   // 8x8 tiles
     // Whitted with max_depth <= 1 has no recursion: the frame-free program
     const int mode = st.pathtrace ? 1 : (st.max_depth <= 1 ? 2 : 0);
@@ -959,11 +966,14 @@ static int render_impl(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const
         HIP_TRY(ctx, hipGetLastError());
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    uint32_t n_deferred = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(h, ctx->d_ctr, kCtrSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    uint32_t* h_deferred = reinterpret_cast<uint32_t*>(h + kCtrSlots);
+    *h_deferred = 0;
     if (defer)
-        HIP_TRY(ctx, hipMemcpyAsync(&n_deferred, ctx->d_gcount, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(h_deferred, ctx->d_gcount, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const uint32_t n_deferred = *h_deferred;
     float kms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
     fill_stats(stats, h, kms, t0);
