@@ -1003,6 +1003,8 @@ int pcl_render_counts(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const 
     return render_impl(ctx, scene, cam, settings, d_out_rgb, stats, counts ? counts : &tmp);
 } PCL_CATCH(ctx)
 
+static int img_buffers(pcl_ctx* ctx, size_t n);
+
 int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_settings* settings, float* out_rgb,
                pcl_stats* stats) try {
     if (!out_rgb) return fail(ctx, PCL_ERR_INVALID, "null output");
@@ -1010,13 +1012,14 @@ int pcl_render(pcl_ctx* ctx, int32_t scene, const pcl_camera* cam, const pcl_set
     if (rc) return rc;
     if (!get_scene(ctx, scene)) return fail(ctx, PCL_ERR_INVALID, "bad scene handle");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
-    const size_t bytes = (size_t)settings->width * settings->height * 3 * sizeof(float);
-    void* d = nullptr;
-    HIP_TRY(ctx, hipMalloc(&d, bytes));
+    const size_t n = (size_t)settings->width * settings->height * 3;
+    // the image on the device: the context's staging buffer, grown on demand
+    // and kept (no allocation per frame)
+    if ((rc = img_buffers(ctx, n))) return rc;
     const double t0 = now_ms();
-    rc = pcl_render_device(ctx, scene, cam, settings, (float*)d, stats);
-    if (!rc && hipMemcpy(out_rgb, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(ctx, PCL_ERR_HIP, "copy image");
-    (void)hipFree(d);
+    rc = pcl_render_device(ctx, scene, cam, settings, ctx->d_img, stats);
+    if (!rc && hipMemcpy(out_rgb, ctx->d_img, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(ctx, PCL_ERR_HIP, "copy image");
     if (!rc && stats) stats->total_ms = now_ms() - t0;
     return rc;
 } PCL_CATCH(ctx)
