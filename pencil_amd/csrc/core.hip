@@ -98,10 +98,12 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene S, const pcl_ray
 }
 
 #ifndef PCL_SHALLOW_WAVES
-#define PCL_SHALLOW_WAVES 4
+#define PCL_SHALLOW_WAVES 3
 #endif
 // Waves per SIMD the register budget allows: Whitted 3, path tracing 4, the
-// frame-free max_depth <= 1 program PCL_SHALLOW_WAVES.
+// frame-free max_depth <= 1 program PCL_SHALLOW_WAVES (with 8x8 tiles per
+// wave, the 1080p 1-spp stress frame 0.838 ms wall at 4 waves, 0.819 at 3,
+// 0.814 at 2, 0.876 at 5).
 template <int PT>
 struct RenderWaves { static constexpr int v = PT == 1 ? 4 : PT == 2 ? PCL_SHALLOW_WAVES : 3; };
 
